@@ -1,0 +1,221 @@
+/*
+ * rtamd.h -- C-ABI of the MI355X ray-traversal library (librtamd.so).
+ *
+ * This is the drop-in boundary for the compute ray tracer's hot path: the
+ * scene arrays, flat BVH node array and RGBA8 target/accumulation images that
+ * the reference binds to descriptor set 0 and dispatches with
+ *     ComputeModel::computeCommand  (src/scene/ComputeModel.cpp:21-25)
+ * over the shader resources/shaders/source/ray-trace-compute.comp.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (RT_OK == 0).  On failure the
+ *     message is kept per context and read with rt_last_error().  No C++
+ *     exception crosses this boundary.
+ *   - Record structs below are byte-identical to the reference's std430
+ *     records (src/ray-tracing/GpuModels.h:26-63, definitions.glsl:1-29).
+ *   - Pointers named *_dev are device (HBM) pointers; everything else is host
+ *     memory.  Streams are hipStream_t passed as void* (NULL = the context's
+ *     own stream).  A context is externally synchronised.
+ */
+#ifndef RTAMD_H
+#define RTAMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+    RT_OK = 0,
+    RT_ERR_INVALID = 1,     /* bad argument or malformed scene             */
+    RT_ERR_HIP = 2,         /* a HIP runtime call failed                    */
+    RT_ERR_NOMEM = 3,       /* allocation failed or HBM budget exceeded     */
+    RT_ERR_UNSUPPORTED = 4, /* valid request this build cannot serve        */
+    RT_ERR_BUSY = 5,        /* server queue full / slot unavailable         */
+    RT_ERR_CANCELLED = 6,   /* request cancelled or deadline expired        */
+    RT_ERR_NOT_FOUND = 7    /* unknown scene / request id                   */
+};
+
+/* Material type ids: definitions.glsl:37-40, GpuModels.h:122-128. */
+enum { RT_MAT_LIGHT = 0, RT_MAT_LAMBERTIAN = 1, RT_MAT_METAL = 2, RT_MAT_GLASS = 3 };
+
+/* ----------------------------------------------------------------- records */
+/* GpuModel::Triangle (GpuModels.h:136-142): vec3 slots at 0/16/32, index @44. */
+typedef struct rt_triangle {
+    float v0[3]; float _pad0;
+    float v1[3]; float _pad1;
+    float v2[3]; uint32_t materialIndex;
+} rt_triangle;                                   /* 48 bytes */
+
+/* GpuModel::Material (GpuModels.h:130-134): type @0, albedo @16. */
+typedef struct rt_material {
+    uint32_t type; uint32_t _pad0[3];
+    float albedo[3]; float _pad1;
+} rt_material;                                   /* 32 bytes */
+
+/* GpuModel::BvhNode (GpuModels.h:151-158): min @0, max @16, left @28,
+ * right @32, objectIndex @36; -1 marks "none". */
+typedef struct rt_bvh_node {
+    float min[3]; float _pad0;
+    float max[3]; int32_t leftNodeIndex;
+    int32_t rightNodeIndex; int32_t objectIndex; int32_t _pad1[2];
+} rt_bvh_node;                                   /* 48 bytes */
+
+/* GpuModel::Light (GpuModels.h:161-167). */
+typedef struct rt_light {
+    uint32_t triangleIndex; float area;
+} rt_light;                                      /* 8 bytes */
+
+/* GpuModel::Sphere (GpuModels.h:144-148): (cx,cy,cz,r) @0, index @16. */
+typedef struct rt_sphere {
+    float s[4]; uint32_t materialIndex; uint32_t _pad[3];
+} rt_sphere;                                     /* 32 bytes */
+
+/* UniformBufferObject (main.cpp:39-47 == ray-trace-compute.comp:8-15). */
+typedef struct rt_ubo {
+    float camPos[3]; float time;
+    uint32_t currentSample; uint32_t numTriangles; uint32_t numLights; uint32_t numSpheres;
+} rt_ubo;                                        /* 32 bytes */
+
+/* Query ray: origin, tmin; direction, tmax.  The shader's hit_bvh uses
+ * tmin = 0.001, tmax = 10000 (ray-trace-compute.comp:264-265). */
+typedef struct rt_ray {
+    float o[3]; float tmin;
+    float d[3]; float tmax;
+} rt_ray;                                        /* 32 bytes */
+
+/* Closest hit: id = triangle index (or num_triangles + sphere index when
+ * spheres are enabled), -1 = miss.  t, u, v as triIntersect returns them
+ * (ray-trace-compute.comp:157-173): u weights v1, v weights v2. */
+typedef struct rt_hit {
+    int32_t id; float t; float u; float v;
+} rt_hit;                                        /* 16 bytes */
+
+/* ------------------------------------------------------------ scene upload */
+typedef struct rt_scene_desc {
+    const rt_triangle* triangles; uint32_t num_triangles;
+    const rt_material* materials; uint32_t num_materials;
+    const rt_bvh_node* nodes;     uint32_t num_nodes;
+    const rt_light*    lights;    uint32_t num_lights;
+    const rt_sphere*   spheres;   uint32_t num_spheres;
+} rt_scene_desc;
+
+/* Facts the library derived from a scene at upload. */
+typedef struct rt_scene_info {
+    uint32_t num_triangles, num_nodes, num_lights, num_spheres, num_materials;
+    uint32_t tree_height;     /* max leaf depth below the root (root = 0)     */
+    uint32_t reachable_leaves;
+    uint32_t enclosing;       /* 1: every box holds its subtree (fast path ok) */
+    uint32_t fast_path;       /* 1: packed ordered traversal is used           */
+    uint64_t device_bytes;    /* HBM held by the resident copy                 */
+} rt_scene_info;
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_scene rt_scene;
+
+/* ---------------------------------------------------------------- context */
+/* Replaces VulkanApplicationContext device/allocator set-up
+ * (src/app-context/VulkanApplicationContext.cpp:47-93). */
+int rt_create(rt_ctx** out, int device);
+int rt_destroy(rt_ctx* ctx);
+const char* rt_last_error(const rt_ctx* ctx);
+const char* rt_status_string(int status);
+
+/* Copies the arrays into HBM, validates indices and the tree, and repacks
+ * nodes/triangles for traversal.  Replaces the six BufferUtils::createBundle
+ * uploads (main.cpp:84-106, Buffer.h:83-102) and ComputeMaterial::init
+ * (ComputeMaterial.cpp:15-26).  Counts are whole-array counts (the shader's
+ * .length()), not the one-element descriptor range of Buffer.h:85.  An empty
+ * scene (no triangles or no nodes) is rejected with RT_ERR_INVALID. */
+int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out);
+int rt_scene_info_get(const rt_scene* scene, rt_scene_info* info);
+/* The validation rt_scene_upload performs, without a device: fills the
+ * tree facts of *info (device_bytes = 0) and, on RT_ERR_INVALID, copies the
+ * reason into err[0..errlen). */
+int rt_scene_check(const rt_scene_desc* desc, rt_scene_info* info, char* err, size_t errlen);
+int rt_scene_release(rt_ctx* ctx, rt_scene* scene);
+
+/* -------------------------------------------------------- closest-hit query */
+enum {
+    RT_TRAVERSE_AUTO = 0,       /* packed ordered traversal when the tree
+                                   encloses its triangles, else reference    */
+    RT_TRAVERSE_PACKED = 1,     /* force packed ordered traversal            */
+    RT_TRAVERSE_REFERENCE = 2   /* hit_bvh's own visiting order and line test */
+};
+typedef struct rt_query_params {
+    uint32_t traversal;   /* RT_TRAVERSE_*                                       */
+    uint32_t max_stack;   /* reference mode only: 0 = unlimited, 16 = shader's
+                             MAX_STACK_DEPTH truncation (ray-trace-compute.comp:262) */
+    uint32_t spheres;     /* 1: also test spheres after the BVH (closest wins)   */
+    uint32_t reserved;
+} rt_query_params;
+
+/* hit_bvh (ray-trace-compute.comp:263-311) over n rays: rays_dev[n] ->
+ * hits_dev[n].  Enqueued on `stream`; returns without waiting. */
+int rt_closest_hit(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays_dev,
+                   uint32_t n, rt_hit* hits_dev, const rt_query_params* params,
+                   void* stream);
+
+/* Per-ray traversal counters of the packed kernel (node visits = 64-byte
+ * node records fetched, tri tests = 48-byte triangle records fetched). */
+int rt_closest_hit_stats(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays_dev,
+                         uint32_t n, rt_hit* hits_dev, uint32_t* node_visits_dev,
+                         uint32_t* tri_tests_dev, void* stream);
+
+/* ------------------------------------------------------------------ render */
+typedef struct rt_render_params {
+    float    vfov_deg;    /* shader hard-codes 30 (ray-trace-compute.comp:358) */
+    uint32_t bounces;     /* shader NUM_BOUNCES = 2 (ray-trace-compute.comp:313) */
+    uint32_t max_stack;   /* 0 = unlimited; 16 = reference truncation (reference
+                             traversal only)                                    */
+    uint32_t spheres;     /* 0 = reference path (hit_bvh, triangles only)       */
+    uint32_t traversal;   /* RT_TRAVERSE_*                                      */
+    uint32_t reserved[3];
+} rt_render_params;
+
+/* One frame of main() (ray-trace-compute.comp:352-380): reads accum_dev
+ * (RGBA8, pitch w*4) and writes target_dev (RGBA8).  The whole w x h image
+ * is rendered (no H/32 truncation of main.cpp:228).  ubo->currentSample
+ * seeds the RNG and weights the progressive average. */
+int rt_render(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo,
+              const rt_render_params* params, uint8_t* target_dev,
+              const uint8_t* accum_dev, uint32_t w, uint32_t h, void* stream);
+
+/* `frames` consecutive frames, currentSample = ubo->currentSample + k, with
+ * the target->accumulation copy of main.cpp:253-261 between frames.  All
+ * frames' paths are traced together; the per-pixel 8-bit running average is
+ * applied in frame order, so the result equals `frames` calls of rt_render
+ * each followed by a copy target->accum.  On return both images hold the
+ * last frame. */
+int rt_render_frames(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo,
+                     const rt_render_params* params, uint32_t frames,
+                     uint8_t* target_dev, uint8_t* accum_dev, uint32_t w,
+                     uint32_t h, void* stream);
+
+/* Traversal count of the last rt_render / rt_render_frames on this context
+ * (one count per hit_bvh call, i.e. per ray). */
+int rt_render_last_rays(const rt_ctx* ctx, uint64_t* primary, uint64_t* secondary);
+
+/* --------------------------------------------------------------- host BVH */
+/* Bvh::createBvh (src/ray-tracing/Bvh.h:141-209): 2n-1 nodes into out[].
+ * Consumes the C library's rand() exactly as the reference does (one call
+ * per node, default seed unless the caller seeded it).  Use
+ * rt_bvh_build_seeded for a builder independent of the process's rand(). */
+int rt_bvh_build(const rt_triangle* tris, uint32_t n, rt_bvh_node* out);
+/* Same tree, with the axis stream of glibc random() seeded by `seed`
+ * (seed 1 == a fresh process), run on `threads` host threads. */
+int rt_bvh_build_seeded(const rt_triangle* tris, uint32_t n, uint32_t seed,
+                        uint32_t threads, rt_bvh_node* out);
+/* Light list rule of RtScene.h:87-96: for each triangle whose material is
+ * a light, Light{i, 0.5*|v0 x v1|}.  Writes up to `cap` lights and returns
+ * the count in *count. */
+int rt_scene_lights(const rt_triangle* tris, uint32_t n, const rt_material* mats,
+                    uint32_t num_mats, rt_light* out, uint32_t cap, uint32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTAMD_H */

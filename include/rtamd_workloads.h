@@ -1,0 +1,71 @@
+/*
+ * rtamd_workloads.h -- synthetic scenes and ray batches of the benchmark
+ * workloads (part of librtamd.so).  These have no counterpart in the
+ * reference, whose only scene is the hard-coded doge scene of
+ * src/ray-tracing/RtScene.h:44-101; they build inputs of the same record
+ * layouts so tests, benchmarks and the server can run without assets.
+ *
+ * Randomness: SplitMix64 (Steele, Lea & Flood, OOPSLA 2014) seeded with the
+ * workload seed; uniform floats take the top 24 bits (x >> 40) * 2^-24;
+ * Gaussians use the Box-Muller transform in double precision.  BVHs come
+ * from rt_bvh_build_seeded(seed 1) -- the reference builder's tree in a
+ * process whose rand() is untouched.  Every array is malloc'ed by the
+ * library and released with rtw_scene_free / free().
+ */
+#ifndef RTAMD_WORKLOADS_H
+#define RTAMD_WORKLOADS_H
+
+#include "rtamd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rtw_scene {
+    rt_triangle* triangles; uint32_t num_triangles;
+    rt_material* materials; uint32_t num_materials;
+    rt_bvh_node* nodes;     uint32_t num_nodes;
+    rt_light*    lights;    uint32_t num_lights;
+    rt_sphere*   spheres;   uint32_t num_spheres;
+} rtw_scene;
+
+/* Random soup: triangle i has centroid U[-1,1]^3 and vertices centroid +
+ * U[-0.1,0.1]^3; materials [0] lambertian 0.7, [1] metal 1.0, [2] glass,
+ * [3] light 4.0 assigned by i mod 4; spheres (+-0.6, 0, 0, r 0.3) with
+ * material 0; lights by the reference rule. */
+int rtw_scene_soup(uint32_t n, uint64_t seed, uint32_t threads, rtw_scene* out);
+
+/* Level-`level` subdivided icosahedron (20 * 4^level triangles) on the unit
+ * sphere, each vertex moved radially by amp * value_noise(seed, freq * p);
+ * one lambertian material, no lights. */
+int rtw_scene_icosphere(uint32_t level, uint64_t seed, float amp, float freq, uint32_t threads, rtw_scene* out);
+
+/* Clustered soup: ceil(n/1024) Gaussian clusters (centre U[-4,4]^3, sigma
+ * 0.15) of equilateral triangles with edge U[0.005,0.03] and random
+ * orientation; materials drawn 60/25/10/5 % lambertian/metal/glass/light. */
+int rtw_scene_clusters(uint32_t n, uint64_t seed, uint32_t threads, rtw_scene* out);
+
+void rtw_scene_free(rtw_scene* s);
+
+/* Camera rays of the shader's model (origin camPos.zxy*(-1,1,1), vfov, unit
+ * directions, tmin 0.001, tmax 10000) for a w x h image, emitted tile by
+ * tile (tile x tile pixels, tiles in row order), pixels row-major inside a
+ * tile; rays [first, first+count) of that order are written. */
+int rtw_rays_camera_tiles(const float cam_pos[3], float vfov_deg, uint32_t w, uint32_t h, uint32_t tile,
+                          uint32_t first, uint32_t count, rt_ray* out);
+
+/* Incoherent rays: origin = radius * normalize(gaussian^3), direction =
+ * normalize(U[-half,half]^3 - origin). */
+int rtw_rays_shell(uint32_t n, uint64_t seed, float radius, float half, rt_ray* out);
+
+/* Uniform rays: origin U[-extent,extent]^3, direction uniform on the sphere. */
+int rtw_rays_box(uint32_t n, uint64_t seed, float extent, rt_ray* out);
+
+/* Log-uniform integer in [lo, hi]: floor(exp(U * ln(hi/lo)) * lo), U from
+ * the seed's first draw (used for scene and batch sizes). */
+uint32_t rtw_log_uniform(uint64_t seed, uint32_t lo, uint32_t hi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTAMD_WORKLOADS_H */

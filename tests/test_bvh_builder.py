@@ -1,0 +1,131 @@
+"""The library's BVH builder (rt_bvh_build*) against the Python restatement of
+Bvh::createBvh (oracle/bvh_ref.py), plus the structural invariants the
+reference algorithm fixes (Bvh.h:141-209)."""
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import bvh_ref
+
+
+def grid_mesh(nx, ny, mat=0):
+    """A planar grid of shared-vertex quads: many triangles share a box minimum,
+    so std::sort's order among equal keys decides the tree."""
+    from rtamd import TRI_DT
+    tris = []
+    for j in range(ny):
+        for i in range(nx):
+            a, b, c, d = (i, j, 0), (i + 1, j, 0), (i + 1, j + 1, 0), (i, j + 1, 0)
+            tris.append((a, b, c))
+            tris.append((a, c, d))
+    out = np.zeros(len(tris), TRI_DT)
+    for k, (a, b, c) in enumerate(tris):
+        out["v0"][k], out["v1"][k], out["v2"][k] = a, b, c
+    out["materialIndex"] = mat
+    return out
+
+
+def same(a, b):
+    """Bit equality of every field (padding bytes are not data)."""
+    if a.dtype != b.dtype or a.shape != b.shape:
+        return False
+    for name in a.dtype.names:
+        x, y = np.ascontiguousarray(a[name]), np.ascontiguousarray(b[name])
+        if not np.array_equal(x.view(np.uint8), y.view(np.uint8)):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 13, 16, 17, 33, 100, 257, 1000])
+def test_builder_matches_restatement_soup(rt, n):
+    sc = rt.scene_soup(n, 7, threads=1)
+    ref = bvh_ref.create_bvh(sc.triangles)
+    assert same(sc.nodes, ref)
+    assert same(rt.bvh_build(sc.triangles, seed=1, threads=4), ref)
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 2), (8, 8), (20, 13)])
+def test_builder_matches_restatement_with_ties(rt, nx, ny):
+    tris = grid_mesh(nx, ny)
+    ref = bvh_ref.create_bvh(tris)
+    assert same(rt.bvh_build(tris, seed=1, threads=1), ref)
+    assert same(rt.bvh_build(tris, seed=1, threads=8), ref)
+
+
+def test_builder_duplicate_triangles(rt):
+    base = rt.scene_soup(40, 3, threads=1).triangles
+    tris = np.concatenate([base, base, base[::-1]])
+    ref = bvh_ref.create_bvh(tris)
+    assert same(rt.bvh_build(tris, seed=1), ref)
+
+
+def test_builder_other_seed_and_heapsort_fallback(rt):
+    # a seeded stream other than 1; 300 equal keys exercise deep partitions
+    from rtamd import TRI_DT
+    tris = np.zeros(300, TRI_DT)
+    tris["v1"][:, 0] = 1
+    tris["v1"][:, 2] = np.arange(300) % 3
+    tris["v2"][:, 1] = 1
+    ref = bvh_ref.create_bvh(tris, bvh_ref.GlibcRandom(99))
+    assert same(rt.bvh_build(tris, seed=99), ref)
+
+
+def test_process_rand_builder_is_seed_one_in_fresh_process():
+    code = (
+        "import sys; sys.path[:0]=[%r, %r];"
+        "import numpy as np, rtamd;"
+        "s=rtamd.scene_soup(500, 11, threads=1);"
+        "a=rtamd.bvh_build(s.triangles);"  # consumes this process's rand()
+        "b=rtamd.bvh_build(s.triangles, seed=1);"
+        "print(int(all(np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8),"
+        " np.ascontiguousarray(b[k]).view(np.uint8)) for k in a.dtype.names)))"
+    ) % (sys.path[0], sys.path[1])
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True).stdout
+    assert out.strip() == "1"
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 1000, 4097])
+def test_builder_invariants(rt, n):
+    sc = rt.scene_soup(n, 5, threads=2)
+    nodes, tris = sc.nodes, sc.triangles
+    assert len(nodes) == 2 * n - 1
+    leaves = nodes[nodes["object"] >= 0]
+    assert len(leaves) == n and sorted(leaves["object"]) == list(range(n))
+    inner = nodes[nodes["object"] < 0]
+    assert np.all(inner["right"] == inner["left"] + 1)            # children adjacent
+    assert np.all(leaves["left"] == -1) and np.all(leaves["right"] == -1)
+    eps = np.float32(1e-4)
+    for node in leaves[:200]:
+        t = tris[node["object"]]
+        v = np.stack([t["v0"], t["v1"], t["v2"]])
+        assert np.array_equal(node["min"], (v.min(0) - eps).astype(np.float32))
+        assert np.array_equal(node["max"], (v.max(0) + eps).astype(np.float32))
+    for node in inner[:500]:
+        l, r = nodes[node["left"]], nodes[node["right"]]
+        assert np.array_equal(node["min"], np.minimum(l["min"], r["min"]))
+        assert np.array_equal(node["max"], np.maximum(l["max"], r["max"]))
+    # creation-order numbering: every child index exceeds its parent's
+    idx = np.nonzero(nodes["object"] < 0)[0]
+    assert np.all(nodes["left"][idx] > idx)
+
+
+def test_light_rule_matches_restatement(rt):
+    sc = rt.scene_soup(1024, 1)
+    ref = bvh_ref.scene_lights(sc.triangles, sc.materials)
+    assert same(sc.lights, ref) and len(ref) == 256
+    # the rule crosses positions, not edges (RtScene.h:93)
+    t = sc.triangles[ref[0]["triangleIndex"]]
+    want = np.float32(np.linalg.norm(np.cross(t["v0"].astype(np.float64), t["v1"].astype(np.float64))) * 0.5)
+    assert abs(float(ref[0]["area"]) - float(want)) <= 1e-6 * max(1.0, float(want))
+
+
+def test_builder_rejects_empty_and_nan(rt):
+    from rtamd import RtError, TRI_DT
+    with pytest.raises(RtError):
+        rt.bvh_build(np.zeros(0, TRI_DT), seed=1)
+    t = rt.scene_soup(4, 1).triangles.copy()
+    t["v1"][2, 0] = np.nan
+    with pytest.raises(RtError):
+        rt.bvh_build(t, seed=1)

@@ -1,0 +1,213 @@
+"""GPU parity of the closest-hit query (hit_bvh, ray-trace-compute.comp:263-311)
+through the C-ABI, against the oracle on the same seeded inputs.
+
+Bar: the packed ordered kernel reproduces the oracle's ordered traversal bit
+for bit (id, t, u, v and per-ray node/triangle counts); against the shader's
+own visiting order the hit id may differ only where the two nearest hits lie
+within 1e-6 in t, and t/u/v are bit-identical whenever the ids agree.  The
+reference-order kernel reproduces the shader's order exactly, ties included.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oscene(hs):
+    return O.Scene(hs.triangles, hs.materials, hs.nodes, hs.lights, hs.spheres)
+
+
+def run(ctx, scene, rays, **kw):
+    import rtamd
+    rd = rtamd.rays_to_device(rays)
+    h = ctx.closest_hit(scene, rd, **kw)
+    torch.cuda.synchronize()
+    return rtamd.hits_from_device(h)
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def assert_bit_equal(got, want):
+    assert np.array_equal(got["id"], want["id"]), np.nonzero(got["id"] != want["id"])[0][:10]
+    for f in ("t", "u", "v"):
+        assert np.array_equal(bits(got[f]), bits(want[f])), f
+
+
+def assert_equal_up_to_ties(sc, rays, got, want):
+    diff = np.nonzero(got["id"] != want["id"])[0]
+    for i in diff:
+        ts = []
+        for j in (int(got["id"][i]), int(want["id"][i])):
+            assert j >= 0, (i, got[i], want[i])
+            ts.append(O.hit_triangle(sc.triangles[j:j + 1], rays["o"][i], rays["d"][i])[3])
+        assert abs(ts[0] - ts[1]) <= 1e-6 * max(1.0, abs(ts[0])), (i, ts)
+    same = got["id"] == want["id"]
+    for f in ("t", "u", "v"):
+        assert np.array_equal(bits(got[f][same]), bits(want[f][same])), f
+    return len(diff)
+
+
+def mixed_rays(rt, n, seed, box=1.2):
+    return np.concatenate([rt.rays_shell(n // 2, seed, 3.0, 0.5), rt.rays_box(n - n // 2, seed + 1, box)])
+
+
+@pytest.fixture(scope="module")
+def soup(rt, gpu_ctx):
+    hs = rt.scene_soup(1024, 1)
+    print("case: soup 1024 upload", flush=True)
+    return hs, gpu_ctx.upload(hs)
+
+
+def test_soup_packed_matches_oracle_ordered(rt, gpu_ctx, soup):
+    hs, scene = soup
+    assert scene.info.fast_path == 1
+    rays = mixed_rays(rt, 65536, 11)
+    print("case: soup packed vs ordered, 65536 rays", flush=True)
+    got = run(gpu_ctx, scene, rays)
+    want, nv, tt, _ = O.closest_hit(oscene(hs), rays, O.TRAV_ORDERED, stats=True, threads=8)
+    assert_bit_equal(got, want)
+    assert (got["id"] >= 0).mean() > 0.1
+    # per-ray work counters equal the oracle's canonical counts
+    hits, gnv, gtt = gpu_ctx.closest_hit_stats(scene, rt.rays_to_device(rays))
+    torch.cuda.synchronize()
+    assert np.array_equal(gnv.cpu().numpy().astype(np.uint32), nv)
+    assert np.array_equal(gtt.cpu().numpy().astype(np.uint32), tt)
+
+
+def test_soup_packed_vs_shader_order(rt, gpu_ctx, soup):
+    hs, scene = soup
+    rays = mixed_rays(rt, 32768, 12)
+    print("case: soup packed vs reference order", flush=True)
+    got = run(gpu_ctx, scene, rays)
+    want = O.closest_hit(oscene(hs), rays, O.TRAV_REFERENCE, threads=8)
+    ties = assert_equal_up_to_ties(oscene(hs), rays, got, want)
+    assert ties <= 2
+
+
+def test_soup_reference_kernel_exact(rt, gpu_ctx, soup):
+    import rtamd
+    hs, scene = soup
+    rays = mixed_rays(rt, 32768, 13)
+    print("case: soup reference kernel", flush=True)
+    got = run(gpu_ctx, scene, rays, traversal=rtamd.TRAVERSE_REFERENCE)
+    want = O.closest_hit(oscene(hs), rays, O.TRAV_REFERENCE, threads=8)
+    assert_bit_equal(got, want)
+
+
+def test_spheres_closest_of_both(rt, gpu_ctx, soup):
+    import rtamd
+    hs, scene = soup
+    rays = np.concatenate([rt.rays_camera_tiles([0, 3, 0], 60.0, 128, 128, 8, 0, 16384), mixed_rays(rt, 8192, 14)])
+    print("case: soup spheres", flush=True)
+    want = O.closest_hit(oscene(hs), rays, O.TRAV_ORDERED, spheres=1, threads=8)
+    assert (want["id"] >= len(hs.triangles)).any()
+    assert_bit_equal(run(gpu_ctx, scene, rays, spheres=1), want)
+    want_ref = O.closest_hit(oscene(hs), rays, O.TRAV_REFERENCE, spheres=1, threads=8)
+    assert_bit_equal(run(gpu_ctx, scene, rays, spheres=1, traversal=rtamd.TRAVERSE_REFERENCE), want_ref)
+
+
+def test_edge_rays(rt, gpu_ctx, soup):
+    """Axis-aligned directions (zero components), origins inside boxes,
+    tiny tmax (all miss), and rays starting on a triangle."""
+    from rtamd import RAY_DT
+    hs, scene = soup
+    r = np.zeros(4096, RAY_DT)
+    rng = np.random.default_rng(5)
+    axes = np.eye(3)[rng.integers(0, 3, 4096)] * rng.choice([-1.0, 1.0], 4096)[:, None]
+    r["o"] = rng.uniform(-1.2, 1.2, (4096, 3))
+    r["d"] = axes
+    r["tmin"] = 0.001
+    r["tmax"] = np.where(np.arange(4096) % 7 == 0, 0.002, 10000.0)
+    t0 = hs.triangles[:512]
+    r["o"][:512] = (t0["v0"] + t0["v1"] + t0["v2"]) / 3          # start on a triangle
+    print("case: soup edge rays", flush=True)
+    want = O.closest_hit(oscene(hs), r, O.TRAV_ORDERED, threads=8)
+    assert_bit_equal(run(gpu_ctx, scene, r), want)
+    ref = O.closest_hit(oscene(hs), r, O.TRAV_REFERENCE, threads=8)
+    assert_equal_up_to_ties(oscene(hs), r, run(gpu_ctx, scene, r), ref)
+
+
+def test_single_triangle_and_empty_batches(rt, gpu_ctx):
+    hs = rt.scene_soup(1, 3)
+    scene = gpu_ctx.upload(hs)
+    t = hs.triangles[0]
+    c = (t["v0"] + t["v1"] + t["v2"]) / 3
+    from rtamd import RAY_DT
+    rays = np.zeros(3, RAY_DT)
+    rays["o"] = c + np.array([0, 0, 2.0])
+    rays["d"] = [[0, 0, -1], [0, 0, 1], [0.001, 0, -1]]
+    rays["tmin"], rays["tmax"] = 0.001, 10000.0
+    print("case: single triangle", flush=True)
+    got = run(gpu_ctx, scene, rays)
+    assert_bit_equal(got, O.closest_hit(oscene(hs), rays, O.TRAV_REFERENCE))
+    # n = 0 is a no-op
+    empty = torch.zeros((0, 8), dtype=torch.float32, device="cuda")
+    gpu_ctx.closest_hit(scene, empty)
+    scene.release()
+
+
+def test_non_enclosing_tree_is_traced_in_shader_order(rt, gpu_ctx):
+    hs = rt.scene_soup(256, 8)
+    leaf = np.nonzero(hs.nodes["object"] >= 0)[0][:40]
+    hs.nodes["max"][leaf] = hs.nodes["min"][leaf] + 0.01     # boxes that miss parts of their triangles
+    scene = gpu_ctx.upload(hs)
+    assert scene.info.fast_path == 0
+    rays = mixed_rays(rt, 16384, 15, box=0.8)
+    print("case: non-enclosing tree", flush=True)
+    assert_bit_equal(run(gpu_ctx, scene, rays), O.closest_hit(oscene(hs), rays, O.TRAV_REFERENCE, threads=8))
+    scene.release()
+
+
+def test_max_stack_16_truncation_mode(rt, gpu_ctx):
+    import rtamd
+    hs = rt.scene_soup(8193, 21)
+    scene = gpu_ctx.upload(hs)
+    rays = mixed_rays(rt, 8192, 16)
+    print("case: max_stack 16 truncation", flush=True)
+    want, _, _, tr = O.closest_hit(oscene(hs), rays, O.TRAV_REFERENCE, max_stack=16, stats=True, threads=8)
+    got = run(gpu_ctx, scene, rays, traversal=rtamd.TRAVERSE_REFERENCE, max_stack=16)
+    assert_bit_equal(got, want)
+    scene.release()
+
+
+def test_icosphere_one_million_rays(rt, gpu_ctx):
+    """The closest-hit query workload: level-6 displaced icosphere, 1M rays
+    (half coherent camera tiles, half incoherent).  Full size: packed kernel
+    against the reference-order kernel; a 32k sample against the oracle."""
+    import rtamd
+    hs = rt.scene_icosphere(6, 2, 0.1, 4.0)
+    assert len(hs.triangles) == 81920
+    scene = gpu_ctx.upload(hs)
+    coh = rt.rays_camera_tiles([0, 3, 0], 60.0, 1024, 1024, 8, 0, 524288)
+    inc = rt.rays_shell(524288, 2024, 3.0, 0.5)
+    rays = np.concatenate([coh, inc])
+    print("case: icosphere 1M packed vs reference kernel", flush=True)
+    got = run(gpu_ctx, scene, rays)
+    ref = run(gpu_ctx, scene, rays, traversal=rtamd.TRAVERSE_REFERENCE)
+    assert (got["id"][524288:] >= 0).all()                  # every shell ray crosses the closed surface
+    diff = np.nonzero(got["id"] != ref["id"])[0]
+    assert len(diff) <= 50
+    same = got["id"] == ref["id"]
+    for f in ("t", "u", "v"):
+        assert np.array_equal(bits(got[f][same]), bits(ref[f][same]))
+    sample = np.concatenate([np.arange(0, 524288, 32), np.arange(524288, 1048576, 32)])
+    print("case: icosphere sample vs oracle", flush=True)
+    want = O.closest_hit(oscene(hs), rays[sample], O.TRAV_ORDERED, threads=8)
+    assert_bit_equal(got[sample], want)
+    assert_equal_up_to_ties(oscene(hs), rays[sample], got[sample],
+                            O.closest_hit(oscene(hs), rays[sample], O.TRAV_REFERENCE, threads=8))
+    scene.release()
+
+
+def test_invalid_scene_rejected_by_upload(rt, gpu_ctx):
+    import rtamd
+    hs = rt.scene_soup(16, 1)
+    hs.nodes["left"][0] = 5000
+    with pytest.raises(rtamd.RtError) as e:
+        gpu_ctx.upload(hs)
+    assert e.value.status == rtamd.RT_ERR_INVALID and "child index" in str(e.value)

@@ -1,0 +1,138 @@
+// rt_common.hpp -- internal declarations of librtamd: packed device
+// layouts, the context and scene objects, error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "rtamd.h"
+
+// The C records must keep the reference's std430 layout (GpuModels.h:26-63).
+static_assert(sizeof(rt_triangle) == 48, "Triangle is 48 bytes");
+static_assert(offsetof(rt_triangle, v1) == 16 && offsetof(rt_triangle, v2) == 32, "Triangle vec3 slots");
+static_assert(offsetof(rt_triangle, materialIndex) == 44, "Triangle.materialIndex @44");
+static_assert(sizeof(rt_material) == 32 && offsetof(rt_material, albedo) == 16, "Material");
+static_assert(sizeof(rt_bvh_node) == 48, "BvhNode is 48 bytes");
+static_assert(offsetof(rt_bvh_node, max) == 16 && offsetof(rt_bvh_node, leftNodeIndex) == 28, "BvhNode min/max/left");
+static_assert(offsetof(rt_bvh_node, rightNodeIndex) == 32 && offsetof(rt_bvh_node, objectIndex) == 36, "BvhNode right/object");
+static_assert(sizeof(rt_light) == 8, "Light");
+static_assert(sizeof(rt_sphere) == 32 && offsetof(rt_sphere, materialIndex) == 16, "Sphere");
+static_assert(sizeof(rt_ubo) == 32 && offsetof(rt_ubo, currentSample) == 16, "UBO");
+static_assert(sizeof(rt_ray) == 32 && sizeof(rt_hit) == 16, "ray/hit");
+
+namespace rtamd {
+
+// ---------------------------------------------------------------------------
+// Packed traversal records.
+//
+// PNode (64 bytes, four 16-byte loads): an internal node of the binary tree
+// holding BOTH children's boxes, so one fetch decides which children to
+// enter.  ref >= 0 names another PNode; ref < 0 names a leaf whose triangle
+// sits in PTri slot ~ref.
+//   a = (c0.lo.x, c0.lo.y, c0.lo.z, c0.hi.x)
+//   b = (c0.hi.y, c0.hi.z, c1.lo.x, c1.lo.y)
+//   c = (c1.lo.z, c1.hi.x, c1.hi.y, c1.hi.z)
+//   d = (ref0, ref1, 0, 0)
+// PTri (48 bytes, three 16-byte loads): the values triIntersect derives from
+// the triangle alone, computed once with the same float operations
+// (a = v0-v1, b = v2-v0, n = cross(b, a), ray-trace-compute.comp:159-162):
+//   a = (v0.x, v0.y, v0.z, e.x)   e = v0 - v1
+//   b = (e.y, e.z, f.x, f.y)      f = v2 - v0
+//   c = (f.z, n.x, n.y, n.z)      n = cross(f, e)
+// ---------------------------------------------------------------------------
+struct alignas(16) PNode { float4 a, b, c; int4 d; };
+struct alignas(16) PTri { float4 a, b, c; };
+static_assert(sizeof(PNode) == 64 && sizeof(PTri) == 48, "packed records");
+
+constexpr int32_t kEmptyRoot = INT32_MIN;  // tree with no reachable triangle
+
+// Bytes each traversal step reads, used for the roofline accounting.
+constexpr uint32_t kNodeBytes = 64;
+constexpr uint32_t kTriBytes = 48;
+
+// Result of repacking a reference node array (host side).
+struct PackedTree {
+    std::vector<PNode> nodes;
+    std::vector<PTri> tris;
+    std::vector<int32_t> tri_id;   // slot -> original triangle index
+    int32_t root = kEmptyRoot;
+    uint32_t height = 0;           // packed internal-node depth (stack bound)
+};
+
+struct SceneCheck {
+    bool standard = true;     // every reachable node is a leaf or an internal node
+    bool enclosing = true;    // boxes hold their subtrees (strict at leaves)
+    uint32_t height = 0;      // max leaf depth of the reference tree
+    uint32_t leaves = 0;
+    std::string error;        // non-empty: the scene is invalid
+};
+
+// Host helpers (rt_scene.cpp).
+SceneCheck check_scene(const rt_scene_desc& d);
+void pack_tree(const rt_scene_desc& d, PackedTree& out);
+
+}  // namespace rtamd
+
+struct rt_scene {
+    rt_ctx* ctx = nullptr;
+    rt_scene_info info{};
+    rt_triangle* d_tris = nullptr;
+    rt_material* d_mats = nullptr;
+    rt_bvh_node* d_nodes = nullptr;
+    rt_light* d_lights = nullptr;
+    rt_sphere* d_spheres = nullptr;
+    rtamd::PNode* d_pnodes = nullptr;
+    rtamd::PTri* d_ptris = nullptr;
+    int32_t* d_ptri_id = nullptr;
+    int32_t root_ref = rtamd::kEmptyRoot;
+    uint32_t packed_height = 0;
+    uint32_t ref_height = 0;
+    bool fast = false;
+};
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cus = 0;
+    // grow-only device workspaces: [0] render path state, [1] reference
+    // traversal stacks, [2] statistics scratch
+    void* ws[3] = {nullptr, nullptr, nullptr};
+    size_t ws_bytes[3] = {0, 0, 0};
+    uint32_t* d_counters = nullptr;   // small zero-filled counter block
+    uint64_t last_primary = 0, last_secondary = 0;
+};
+
+namespace rtamd {
+
+int set_error(rt_ctx* ctx, int code, const std::string& msg);
+int hip_error(rt_ctx* ctx, hipError_t e, const char* what);
+enum { kWsRender = 0, kWsStack = 1, kWsStats = 2 };
+int ensure_workspace(rt_ctx* ctx, int slot, size_t bytes);
+
+#define RT_HIP(ctx, call)                                              \
+    do {                                                               \
+        hipError_t _e = (call);                                        \
+        if (_e != hipSuccess) return ::rtamd::hip_error((ctx), _e, #call); \
+    } while (0)
+
+// Launchers (rt_trace.hip).
+struct TraceArgs {
+    const rt_scene* scene;
+    const rt_ray* rays;
+    uint32_t n;                 // used when n_dev is null
+    const uint32_t* n_dev;      // device-side ray count (render queues)
+    rt_hit* hits;
+    uint32_t spheres;
+    uint32_t* node_visits;      // optional per-ray counters
+    uint32_t* tri_tests;
+};
+int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
+int launch_trace_reference(rt_ctx* ctx, const TraceArgs& a, uint32_t max_stack, hipStream_t s);
+int launch_fill_u32(rt_ctx* ctx, uint32_t* p, uint32_t value, uint32_t count, hipStream_t s);
+
+}  // namespace rtamd

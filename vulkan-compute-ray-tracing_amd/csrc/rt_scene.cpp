@@ -1,0 +1,334 @@
+// rt_scene.cpp -- scene validation, repacking and upload to HBM.
+//
+// Replaces the reference's upload of the RtScene arrays into per-swapchain
+// storage buffers (main.cpp:84-106, Buffer.h:83-102) and the descriptor
+// binding of ComputeMaterial (ComputeMaterial.cpp:15-26, Material.cpp:258-322).
+// Unlike Buffer.h:85, each array is bound with its whole length.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "rt_common.hpp"
+#include "rt_math.hpp"
+
+#pragma clang fp contract(off)
+
+namespace rtamd {
+
+namespace {
+
+inline bool finite3(const float* v) {
+    return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+
+// Leaf test of the packed traversal's precondition: the triangle's box lies
+// strictly inside the node box, so no ray along a box face reaches it.
+bool leaf_encloses(const rt_bvh_node& n, const rt_triangle& t) {
+    if (!finite3(n.min) || !finite3(n.max) || !finite3(t.v0) || !finite3(t.v1) || !finite3(t.v2)) return false;
+    for (int k = 0; k < 3; ++k) {
+        float lo = std::min(std::min(t.v0[k], t.v1[k]), t.v2[k]);
+        float hi = std::max(std::max(t.v0[k], t.v1[k]), t.v2[k]);
+        if (!(n.min[k] < lo) || !(hi < n.max[k])) return false;
+    }
+    return true;
+}
+
+bool box_contains(const rt_bvh_node& p, const rt_bvh_node& c) {
+    if (!finite3(p.min) || !finite3(p.max) || !finite3(c.min) || !finite3(c.max)) return false;
+    for (int k = 0; k < 3; ++k)
+        if (!(p.min[k] <= c.min[k]) || !(c.max[k] <= p.max[k])) return false;
+    return true;
+}
+
+}  // namespace
+
+SceneCheck check_scene(const rt_scene_desc& d) {
+    SceneCheck c;
+    auto fail = [&](const std::string& m) { c.error = m; return c; };
+    if (d.num_triangles == 0 || !d.triangles) return fail("scene has no triangles (Bvh.h:77 is undefined for an empty list)");
+    if (d.num_nodes == 0 || !d.nodes) return fail("scene has no BVH nodes");
+    if (d.num_materials == 0 || !d.materials) return fail("scene has no materials");
+    if (d.num_lights && !d.lights) return fail("lights pointer is null");
+    if (d.num_spheres && !d.spheres) return fail("spheres pointer is null");
+    for (uint32_t i = 0; i < d.num_materials; ++i)
+        if (d.materials[i].type > 3u) return fail("material " + std::to_string(i) + " has type > 3");
+    for (uint32_t i = 0; i < d.num_triangles; ++i)
+        if (d.triangles[i].materialIndex >= d.num_materials)
+            return fail("triangle " + std::to_string(i) + " names material out of range");
+    for (uint32_t i = 0; i < d.num_lights; ++i)
+        if (d.lights[i].triangleIndex >= d.num_triangles)
+            return fail("light " + std::to_string(i) + " names triangle out of range");
+    for (uint32_t i = 0; i < d.num_spheres; ++i)
+        if (d.spheres[i].materialIndex >= d.num_materials)
+            return fail("sphere " + std::to_string(i) + " names material out of range");
+    const int32_t nn = (int32_t)d.num_nodes, nt = (int32_t)d.num_triangles;
+    for (int32_t i = 0; i < nn; ++i) {
+        const rt_bvh_node& n = d.nodes[i];
+        if (n.leftNodeIndex < -1 || n.leftNodeIndex >= nn || n.rightNodeIndex < -1 || n.rightNodeIndex >= nn)
+            return fail("node " + std::to_string(i) + " has a child index out of range");
+        if (n.objectIndex < -1 || n.objectIndex >= nt)
+            return fail("node " + std::to_string(i) + " has an object index out of range");
+    }
+    // Walk from the root: every node may be reached once (a tree).
+    std::vector<uint8_t> seen(d.num_nodes, 0);
+    std::vector<std::pair<int32_t, uint32_t>> st;
+    st.emplace_back(0, 0u);
+    while (!st.empty()) {
+        auto [i, depth] = st.back();
+        st.pop_back();
+        if (seen[i]) return fail("node " + std::to_string(i) + " is reachable twice: the node array is not a tree");
+        seen[i] = 1;
+        const rt_bvh_node& n = d.nodes[i];
+        const bool has_kids = n.leftNodeIndex != -1 || n.rightNodeIndex != -1;
+        if (n.objectIndex != -1) {
+            if (has_kids) c.standard = false;
+            c.leaves++;
+            c.height = std::max(c.height, depth);
+            if (!leaf_encloses(n, d.triangles[n.objectIndex])) c.enclosing = false;
+        }
+        for (int32_t ch : {n.leftNodeIndex, n.rightNodeIndex}) {
+            if (ch == -1) continue;
+            if (!box_contains(n, d.nodes[ch])) c.enclosing = false;
+            st.emplace_back(ch, depth + 1);
+        }
+    }
+    return c;
+}
+
+void pack_tree(const rt_scene_desc& d, PackedTree& out) {
+    const uint32_t nn = d.num_nodes;
+    out.nodes.clear();
+    out.tris.clear();
+    out.tri_id.clear();
+    out.nodes.reserve(nn / 2 + 1);
+    out.tris.reserve(d.num_triangles);
+    out.tri_id.reserve(d.num_triangles);
+
+    // has_tri[i]: the subtree under node i holds at least one triangle.
+    // Children may carry any index, so resolve it with a post-order walk.
+    std::vector<uint8_t> has_tri(nn, 0);
+    {
+        std::vector<std::pair<int32_t, bool>> st;
+        st.emplace_back(0, false);
+        while (!st.empty()) {
+            auto [i, done] = st.back();
+            st.pop_back();
+            const rt_bvh_node& n = d.nodes[i];
+            if (done) {
+                bool h = n.objectIndex != -1;
+                if (n.leftNodeIndex != -1 && has_tri[n.leftNodeIndex]) h = true;
+                if (n.rightNodeIndex != -1 && has_tri[n.rightNodeIndex]) h = true;
+                has_tri[i] = h;
+                continue;
+            }
+            st.emplace_back(i, true);
+            if (n.leftNodeIndex != -1) st.emplace_back(n.leftNodeIndex, false);
+            if (n.rightNodeIndex != -1) st.emplace_back(n.rightNodeIndex, false);
+        }
+    }
+
+    auto add_leaf = [&](int32_t obj) -> int32_t {
+        const rt_triangle& t = d.triangles[obj];
+        f3 v0 = ld3(t.v0), v1 = ld3(t.v1), v2 = ld3(t.v2);
+        f3 e = v0 - v1;        // triIntersect's a
+        f3 f = v2 - v0;        // triIntersect's b
+        f3 n = cross(f, e);    // n = cross(b, a)
+        PTri p;
+        p.a = make_float4(v0.x, v0.y, v0.z, e.x);
+        p.b = make_float4(e.y, e.z, f.x, f.y);
+        p.c = make_float4(f.z, n.x, n.y, n.z);
+        int32_t slot = (int32_t)out.tris.size();
+        out.tris.push_back(p);
+        out.tri_id.push_back(obj);
+        return ~slot;
+    };
+
+    // Follow single-child chains down to the node that decides a subtree.
+    auto resolve = [&](int32_t i) -> int32_t {
+        for (;;) {
+            const rt_bvh_node& n = d.nodes[i];
+            if (n.objectIndex != -1) return i;
+            int32_t l = n.leftNodeIndex, r = n.rightNodeIndex;
+            bool hl = l != -1 && has_tri[l], hr = r != -1 && has_tri[r];
+            if (hl && hr) return i;
+            i = hl ? l : r;
+        }
+    };
+
+    if (!has_tri[0]) { out.root = kEmptyRoot; out.height = 0; return; }
+
+    // Pre-order emission with an explicit stack; each frame fills one
+    // child slot of an already-emitted PNode.
+    struct Frame { int32_t src; int32_t parent; int slot; uint32_t depth; };
+    std::vector<Frame> st;
+    int32_t root_src = resolve(0);
+    st.push_back({root_src, -1, 0, 1});
+    out.height = 0;
+    while (!st.empty()) {
+        Frame f = st.back();
+        st.pop_back();
+        const rt_bvh_node& n = d.nodes[f.src];
+        int32_t ref;
+        if (n.objectIndex != -1) {
+            ref = add_leaf(n.objectIndex);
+        } else {
+            ref = (int32_t)out.nodes.size();
+            out.nodes.push_back(PNode{});
+            out.height = std::max(out.height, f.depth);
+            int32_t l = resolve(n.leftNodeIndex), r = resolve(n.rightNodeIndex);
+            const rt_bvh_node& L = d.nodes[n.leftNodeIndex];   // boxes of the direct children
+            const rt_bvh_node& R = d.nodes[n.rightNodeIndex];
+            PNode& p = out.nodes[ref];
+            p.a = make_float4(L.min[0], L.min[1], L.min[2], L.max[0]);
+            p.b = make_float4(L.max[1], L.max[2], R.min[0], R.min[1]);
+            p.c = make_float4(R.min[2], R.max[0], R.max[1], R.max[2]);
+            p.d = make_int4(0, 0, 0, 0);
+            // right pushed first so the left subtree follows its parent
+            st.push_back({r, ref, 1, f.depth + 1});
+            st.push_back({l, ref, 0, f.depth + 1});
+        }
+        if (f.parent < 0) out.root = ref;
+        else if (f.slot == 0) out.nodes[f.parent].d.x = ref;
+        else out.nodes[f.parent].d.y = ref;
+    }
+}
+
+}  // namespace rtamd
+
+using namespace rtamd;
+
+namespace {
+constexpr uint32_t kMaxPackedHeight = 64;
+
+template <typename T>
+int upload_array(rt_ctx* ctx, T** dst, const T* src, uint32_t n, uint64_t& bytes) {
+    *dst = nullptr;
+    if (!n) return RT_OK;
+    size_t b = sizeof(T) * (size_t)n;
+    RT_HIP(ctx, hipMalloc((void**)dst, b));
+    RT_HIP(ctx, hipMemcpy(*dst, src, b, hipMemcpyHostToDevice));
+    bytes += b;
+    return RT_OK;
+}
+
+void free_scene(rt_scene* s) {
+    if (!s) return;
+    for (void* p : {(void*)s->d_tris, (void*)s->d_mats, (void*)s->d_nodes, (void*)s->d_lights,
+                    (void*)s->d_spheres, (void*)s->d_pnodes, (void*)s->d_ptris, (void*)s->d_ptri_id})
+        if (p) (void)hipFree(p);
+    delete s;
+}
+}  // namespace
+
+extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
+    if (!ctx) return RT_ERR_INVALID;
+    if (!desc || !out) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: null argument");
+    *out = nullptr;
+    SceneCheck chk;
+    try {
+        chk = check_scene(*desc);
+    } catch (const std::exception& e) {
+        return set_error(ctx, RT_ERR_NOMEM, std::string("rt_scene_upload: ") + e.what());
+    }
+    if (!chk.error.empty()) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: " + chk.error);
+    RT_HIP(ctx, hipSetDevice(ctx->device));
+
+    PackedTree packed;
+    bool fast = chk.standard && chk.enclosing;
+    if (fast) {
+        try {
+            pack_tree(*desc, packed);
+        } catch (const std::exception& e) {
+            return set_error(ctx, RT_ERR_NOMEM, std::string("rt_scene_upload: ") + e.what());
+        }
+        if (packed.height > kMaxPackedHeight) fast = false;
+    }
+
+    rt_scene* s = new rt_scene();
+    s->ctx = ctx;
+    uint64_t bytes = 0;
+    int rc = RT_OK;
+    do {
+        if ((rc = upload_array(ctx, &s->d_tris, desc->triangles, desc->num_triangles, bytes))) break;
+        if ((rc = upload_array(ctx, &s->d_mats, desc->materials, desc->num_materials, bytes))) break;
+        if ((rc = upload_array(ctx, &s->d_nodes, desc->nodes, desc->num_nodes, bytes))) break;
+        if ((rc = upload_array(ctx, &s->d_lights, desc->lights, desc->num_lights, bytes))) break;
+        if ((rc = upload_array(ctx, &s->d_spheres, desc->spheres, desc->num_spheres, bytes))) break;
+        if (fast) {
+            if ((rc = upload_array(ctx, &s->d_pnodes, packed.nodes.data(), (uint32_t)packed.nodes.size(), bytes))) break;
+            if ((rc = upload_array(ctx, &s->d_ptris, packed.tris.data(), (uint32_t)packed.tris.size(), bytes))) break;
+            if ((rc = upload_array(ctx, &s->d_ptri_id, packed.tri_id.data(), (uint32_t)packed.tri_id.size(), bytes))) break;
+        }
+    } while (0);
+    if (rc) {
+        free_scene(s);
+        return rc;
+    }
+    s->root_ref = fast ? packed.root : kEmptyRoot;
+    s->packed_height = fast ? packed.height : 0;
+    s->ref_height = chk.height;
+    s->fast = fast;
+    rt_scene_info& in = s->info;
+    in.num_triangles = desc->num_triangles;
+    in.num_nodes = desc->num_nodes;
+    in.num_lights = desc->num_lights;
+    in.num_spheres = desc->num_spheres;
+    in.num_materials = desc->num_materials;
+    in.tree_height = chk.height;
+    in.reachable_leaves = chk.leaves;
+    in.enclosing = chk.enclosing ? 1u : 0u;
+    in.fast_path = fast ? 1u : 0u;
+    in.device_bytes = bytes;
+    *out = s;
+    return RT_OK;
+}
+
+extern "C" int rt_scene_check(const rt_scene_desc* desc, rt_scene_info* info, char* err, size_t errlen) {
+    if (err && errlen) err[0] = 0;
+    if (!desc || !info) return RT_ERR_INVALID;
+    SceneCheck chk;
+    try {
+        chk = check_scene(*desc);
+    } catch (const std::exception&) {
+        return RT_ERR_NOMEM;
+    }
+    if (!chk.error.empty()) {
+        if (err && errlen) {
+            std::strncpy(err, chk.error.c_str(), errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        return RT_ERR_INVALID;
+    }
+    bool fast = chk.standard && chk.enclosing;
+    if (fast) {
+        PackedTree packed;
+        pack_tree(*desc, packed);
+        if (packed.height > kMaxPackedHeight) fast = false;
+    }
+    std::memset(info, 0, sizeof *info);
+    info->num_triangles = desc->num_triangles;
+    info->num_nodes = desc->num_nodes;
+    info->num_lights = desc->num_lights;
+    info->num_spheres = desc->num_spheres;
+    info->num_materials = desc->num_materials;
+    info->tree_height = chk.height;
+    info->reachable_leaves = chk.leaves;
+    info->enclosing = chk.enclosing ? 1u : 0u;
+    info->fast_path = fast ? 1u : 0u;
+    return RT_OK;
+}
+
+extern "C" int rt_scene_info_get(const rt_scene* scene, rt_scene_info* info) {
+    if (!scene || !info) return RT_ERR_INVALID;
+    *info = scene->info;
+    return RT_OK;
+}
+
+extern "C" int rt_scene_release(rt_ctx* ctx, rt_scene* scene) {
+    if (!ctx) return RT_ERR_INVALID;
+    if (!scene) return RT_OK;
+    if (scene->ctx != ctx) return set_error(ctx, RT_ERR_INVALID, "rt_scene_release: scene belongs to another context");
+    (void)hipSetDevice(ctx->device);
+    free_scene(scene);
+    return RT_OK;
+}

@@ -1,0 +1,320 @@
+// rt_trace.hip -- closest-hit traversal kernels for gfx950 (wave64).
+//
+// Two kernels compute hit_bvh (ray-trace-compute.comp:263-311):
+//
+//  * k_trace_packed: ordered traversal of the repacked tree (64-byte nodes
+//    carrying both children's boxes, 48-byte triangles with the edge vectors
+//    and normal precomputed).  Children are tested against the live interval
+//    [tmin, closest], the nearer is entered and the farther deferred on a
+//    per-lane stack in LDS.  The triangle test repeats triIntersect /
+//    hit_triangle operation for operation, so t, u, v and the accept
+//    decision are bit-identical to the shader's; the box test is only a
+//    conservative filter (trees whose boxes enclose their subtrees, checked
+//    at upload), so the closest hit equals the shader's up to exact ties.
+//
+//  * k_trace_reference: the shader's own visiting order -- a line test with
+//    six divisions per box (intersectAABB, :250-258), both children pushed
+//    unconditionally, right popped first, optional MAX_STACK_DEPTH guard.
+//    Used for trees that do not enclose their subtrees, for the 16-entry
+//    truncation mode, and as a tie-exact mode.
+#include "rt_common.hpp"
+#include "rt_math.hpp"
+
+#pragma clang fp contract(off)
+
+namespace rtamd {
+
+namespace {
+
+struct RayIn {
+    f3 o, d;
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ RayIn load_ray(const rt_ray* rays, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(rays) + 2 * (size_t)i;
+    float4 a = p[0], b = p[1];
+    return RayIn{f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, a.w, b.w};
+}
+
+__device__ __forceinline__ void store_hit(rt_hit* hits, uint32_t i, int32_t id, float t, float u, float v) {
+    float4 h = make_float4(__int_as_float(id), t, u, v);
+    reinterpret_cast<float4*>(hits)[i] = h;
+}
+
+// Conservative slab test of one box against [tmin, tmax].  Rounding of the
+// reciprocal form is absorbed by a relative slack of 2^-16 (far larger than
+// the few-ulp error), so a box that holds an accepted hit is never culled.
+__device__ __forceinline__ bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz,
+                                     const f3& o, const f3& inv, float tmin, float tmax, float& tnear) {
+    float tx0 = (lox - o.x) * inv.x, tx1 = (hix - o.x) * inv.x;
+    float ty0 = (loy - o.y) * inv.y, ty1 = (hiy - o.y) * inv.y;
+    float tz0 = (loz - o.z) * inv.z, tz1 = (hiz - o.z) * inv.z;
+    float nearv = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    float farv = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    tnear = nearv;
+    return nearv <= farv + (fabsf(farv) + fabsf(nearv)) * 0x1p-16f;
+}
+
+// triIntersect + hit_triangle's accept test on a packed triangle.
+__device__ __forceinline__ bool tri_test(const PTri& tr, const f3& o, const f3& d, float tmin, float tbest,
+                                         float& t, float& u, float& v) {
+    f3 v0{tr.a.x, tr.a.y, tr.a.z};
+    f3 a{tr.a.w, tr.b.x, tr.b.y};
+    f3 b{tr.b.z, tr.b.w, tr.c.x};
+    f3 n{tr.c.y, tr.c.z, tr.c.w};
+    f3 p = v0 - o;
+    f3 q = cross(p, d);
+    float idet = 1.0f / dot(d, n);
+    u = dot(q, b) * idet;
+    v = dot(q, a) * idet;
+    t = dot(n, p) * idet;
+    bool inside = !(u < 0.0f || u > 1.0f || v < 0.0f || (u + v) > 1.0f);
+    return inside && t > tmin && t < tbest;
+}
+
+// hit_sphere (ray-trace-compute.comp:175-203): returns the accepted root
+// and whether it is the far one (backFaceInt).
+__device__ __forceinline__ bool sphere_test(const rt_sphere& s, const f3& o, const f3& d, float tmin, float tmax,
+                                            float& root, int& back) {
+    f3 center{s.s[0], s.s[1], s.s[2]};
+    float radius = s.s[3];
+    f3 oc = o - center;
+    float a = dot(d, d);
+    float half_b = dot(oc, d);
+    float c = dot(oc, oc) - radius * radius;
+    float disc = half_b * half_b - a * c;
+    if (disc < 0.0f) return false;
+    float sqrtd = sqrtf(disc);
+    back = 0;
+    root = (-half_b - sqrtd) / a;
+    if (root < tmin || tmax < root) {
+        root = (-half_b + sqrtd) / a;
+        back = 1;
+        if (root < tmin || tmax < root) return false;
+    }
+    return true;
+}
+
+template <int BLOCK, int STACK, bool STATS>
+__global__ __launch_bounds__(BLOCK) void k_trace_packed(
+    const PNode* __restrict__ nodes, const PTri* __restrict__ tris, const int32_t* __restrict__ tri_id,
+    int32_t root, const rt_ray* __restrict__ rays, uint32_t n_host, const uint32_t* __restrict__ n_dev,
+    rt_hit* __restrict__ hits, const rt_sphere* __restrict__ spheres, uint32_t nspheres, uint32_t ntri,
+    uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests) {
+    __shared__ int32_t stk[STACK * BLOCK];
+    const uint32_t n = n_dev ? *n_dev : n_host;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = blockIdx.x * BLOCK + tid; i < n; i += gridDim.x * BLOCK) {
+        RayIn r = load_ray(rays, i);
+        const f3 inv{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
+        float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
+        int32_t best = -1;
+        uint32_t nv = 0, nt = 0;
+        int sp = 0;
+        int32_t cur = root;
+        if (cur != kEmptyRoot) {
+            for (;;) {
+                if (cur >= 0) {
+                    const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+                    float4 a = np[0], b = np[1], c = np[2];
+                    int4 dd = reinterpret_cast<const int4*>(np)[3];
+                    if (STATS) nv++;
+                    float tn0, tn1;
+                    bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, r.o, inv, r.tmin, best_t, tn0);
+                    bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, r.o, inv, r.tmin, best_t, tn1);
+                    if (h0 && h1) {
+                        bool swap = tn1 < tn0;
+                        int32_t nearc = swap ? dd.y : dd.x;
+                        int32_t farc = swap ? dd.x : dd.y;
+                        stk[sp * BLOCK + tid] = farc;
+                        sp++;
+                        cur = nearc;
+                        continue;
+                    }
+                    if (h0) { cur = dd.x; continue; }
+                    if (h1) { cur = dd.y; continue; }
+                } else {
+                    const int32_t slot = ~cur;
+                    const float4* tp = reinterpret_cast<const float4*>(tris + slot);
+                    PTri tr{tp[0], tp[1], tp[2]};
+                    if (STATS) nt++;
+                    float t, u, v;
+                    if (tri_test(tr, r.o, r.d, r.tmin, best_t, t, u, v)) {
+                        best_t = t; bu = u; bv = v; best = slot;
+                    }
+                }
+                if (sp == 0) break;
+                sp--;
+                cur = stk[sp * BLOCK + tid];
+            }
+        }
+        int32_t id = best >= 0 ? tri_id[best] : -1;
+        if (nspheres) {
+            float closest = best >= 0 ? best_t : r.tmax;
+            for (uint32_t j = 0; j < nspheres; ++j) {
+                float root_t; int back;
+                if (sphere_test(spheres[j], r.o, r.d, r.tmin, closest, root_t, back)) {
+                    closest = root_t; best_t = root_t; id = (int32_t)(ntri + j); bu = (float)back; bv = 0.0f;
+                }
+            }
+        }
+        if (id >= 0) store_hit(hits, i, id, best_t, bu, bv);
+        else store_hit(hits, i, -1, 0.0f, 0.0f, 0.0f);
+        if (STATS) { node_visits[i] = nv; tri_tests[i] = nt; }
+    }
+}
+
+// intersectAABB (ray-trace-compute.comp:250-258) with GLSL min/max.
+__device__ __forceinline__ void aabb_line(const rt_bvh_node& nd, const f3& o, const f3& d, float& tn, float& tf) {
+    f3 tMin = (ld3(nd.min) - o) / d;
+    f3 tMax = (ld3(nd.max) - o) / d;
+    f3 t1 = gmin(tMin, tMax);
+    f3 t2 = gmax(tMin, tMax);
+    tn = gmax(gmax(t1.x, t1.y), t1.z);
+    tf = gmin(gmin(t2.x, t2.y), t2.z);
+}
+
+// hit_triangle on the original record: identical arithmetic to tri_test.
+__device__ __forceinline__ bool tri_test_ref(const rt_triangle& tr, const f3& o, const f3& d, float tmin, float tmax,
+                                             float& t, float& u, float& v) {
+    f3 v0 = ld3(tr.v0), v1 = ld3(tr.v1), v2 = ld3(tr.v2);
+    f3 a = v0 - v1;
+    f3 b = v2 - v0;
+    f3 p = v0 - o;
+    f3 n = cross(b, a);
+    f3 q = cross(p, d);
+    float idet = 1.0f / dot(d, n);
+    u = dot(q, b) * idet;
+    v = dot(q, a) * idet;
+    t = dot(n, p) * idet;
+    bool inside = !(u < 0.0f || u > 1.0f || v < 0.0f || (u + v) > 1.0f);
+    return inside && t > tmin && t < tmax;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_reference(
+    const rt_bvh_node* __restrict__ nodes, const rt_triangle* __restrict__ tris, const rt_ray* __restrict__ rays,
+    uint32_t n_host, const uint32_t* __restrict__ n_dev, rt_hit* __restrict__ hits, int32_t* __restrict__ stack,
+    uint32_t cap, uint32_t max_stack, const rt_sphere* __restrict__ spheres, uint32_t nspheres, uint32_t ntri) {
+    const uint32_t n = n_dev ? *n_dev : n_host;
+    const uint32_t stride = gridDim.x * BLOCK;
+    const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
+    for (uint32_t i = g; i < n; i += stride) {
+        RayIn r = load_ray(rays, i);
+        float closest = r.tmax, bu = 0.0f, bv = 0.0f;
+        int32_t best = -1;
+        uint32_t sp = 0;
+        stack[(size_t)sp * stride + g] = 0;
+        sp++;
+        while (sp > 0 && (max_stack == 0u || sp < max_stack)) {
+            sp--;
+            int32_t cur = stack[(size_t)sp * stride + g];
+            if (cur == -1) continue;
+            const rt_bvh_node nd = nodes[cur];
+            float tn, tf;
+            aabb_line(nd, r.o, r.d, tn, tf);
+            if (tn > tf) continue;
+            if (nd.objectIndex != -1) {
+                float t, u, v;
+                if (tri_test_ref(tris[nd.objectIndex], r.o, r.d, r.tmin, closest, t, u, v)) {
+                    closest = t; bu = u; bv = v; best = nd.objectIndex;
+                }
+            }
+            if (sp + 2 > cap) break;  // unreachable: cap covers the tree height
+            stack[(size_t)sp * stride + g] = nd.leftNodeIndex;
+            sp++;
+            stack[(size_t)sp * stride + g] = nd.rightNodeIndex;
+            sp++;
+        }
+        int32_t id = best;
+        float best_t = closest;
+        if (nspheres) {
+            float cl = best >= 0 ? closest : r.tmax;
+            for (uint32_t j = 0; j < nspheres; ++j) {
+                float root_t; int back;
+                if (sphere_test(spheres[j], r.o, r.d, r.tmin, cl, root_t, back)) {
+                    cl = root_t; best_t = root_t; id = (int32_t)(ntri + j); bu = (float)back; bv = 0.0f;
+                }
+            }
+        }
+        if (id >= 0) store_hit(hits, i, id, best_t, bu, bv);
+        else store_hit(hits, i, -1, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t value, uint32_t count) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = value;
+}
+
+constexpr int kBlock = 256;
+
+template <int STACK>
+int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, uint32_t blocks, bool stats) {
+    const rt_scene* sc = a.scene;
+    if (stats)
+        hipLaunchKernelGGL((k_trace_packed<kBlock, STACK, true>), dim3(blocks), dim3(kBlock), 0, s, sc->d_pnodes,
+                           sc->d_ptris, sc->d_ptri_id, sc->root_ref, a.rays, a.n, a.n_dev, a.hits, sc->d_spheres,
+                           a.spheres ? sc->info.num_spheres : 0u, sc->info.num_triangles, a.node_visits, a.tri_tests);
+    else
+        hipLaunchKernelGGL((k_trace_packed<kBlock, STACK, false>), dim3(blocks), dim3(kBlock), 0, s, sc->d_pnodes,
+                           sc->d_ptris, sc->d_ptri_id, sc->root_ref, a.rays, a.n, a.n_dev, a.hits, sc->d_spheres,
+                           a.spheres ? sc->info.num_spheres : 0u, sc->info.num_triangles, a.node_visits, a.tri_tests);
+    RT_HIP(ctx, hipGetLastError());
+    return RT_OK;
+}
+
+}  // namespace
+
+int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s) {
+    const rt_scene* sc = a.scene;
+    if (!sc->fast) return set_error(ctx, RT_ERR_UNSUPPORTED, "packed traversal needs a tree that encloses its triangles");
+    const bool stats = a.node_visits && a.tri_tests;
+    // grid: enough blocks to fill every CU several times; a grid-stride loop
+    // covers the rest (and device-side counts from the render queues).
+    uint32_t blocks = (uint32_t)ctx->num_cus * 8u;
+    if (!a.n_dev) {
+        uint32_t need = (a.n + kBlock - 1) / kBlock;
+        if (need < blocks) blocks = need;
+        if (blocks == 0) return RT_OK;
+    }
+    const uint32_t h = sc->packed_height;
+    if (h <= 16) return launch_packed_t<16>(ctx, a, s, blocks, stats);
+    if (h <= 24) return launch_packed_t<24>(ctx, a, s, blocks, stats);
+    if (h <= 32) return launch_packed_t<32>(ctx, a, s, blocks, stats);
+    if (h <= 48) return launch_packed_t<48>(ctx, a, s, blocks, stats);
+    if (h <= 64) return launch_packed_t<64>(ctx, a, s, blocks, stats);
+    return set_error(ctx, RT_ERR_UNSUPPORTED, "packed tree deeper than 64 levels");
+}
+
+int launch_trace_reference(rt_ctx* ctx, const TraceArgs& a, uint32_t max_stack, hipStream_t s) {
+    const rt_scene* sc = a.scene;
+    uint32_t blocks = (uint32_t)ctx->num_cus * 8u;
+    if (!a.n_dev) {
+        uint32_t need = (a.n + kBlock - 1) / kBlock;
+        if (need < blocks) blocks = need;
+        if (blocks == 0) return RT_OK;
+    }
+    // Line-test DFS holds at most (height + 1) entries; the guard variant
+    // never exceeds max_stack.
+    uint32_t cap = max_stack ? max_stack : sc->ref_height + 3u;
+    size_t bytes = (size_t)cap * blocks * kBlock * sizeof(int32_t);
+    int rc = ensure_workspace(ctx, kWsStack, bytes);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_trace_reference<kBlock>), dim3(blocks), dim3(kBlock), 0, s, sc->d_nodes, sc->d_tris, a.rays,
+                       a.n, a.n_dev, a.hits, (int32_t*)ctx->ws[kWsStack], cap, max_stack, sc->d_spheres,
+                       a.spheres ? sc->info.num_spheres : 0u, sc->info.num_triangles);
+    RT_HIP(ctx, hipGetLastError());
+    return RT_OK;
+}
+
+int launch_fill_u32(rt_ctx* ctx, uint32_t* p, uint32_t value, uint32_t count, hipStream_t s) {
+    if (!count) return RT_OK;
+    uint32_t blocks = (count + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, value, count);
+    RT_HIP(ctx, hipGetLastError());
+    return RT_OK;
+}
+
+}  // namespace rtamd

@@ -1,0 +1,398 @@
+"""ctypes binding of librtamd.so (include/rtamd.h, include/rtamd_workloads.h).
+
+The library is the product; this module only marshals arguments.  Device
+buffers are torch tensors on a ROCm device (torch supplies HBM and streams);
+host records are numpy arrays with the reference's std430 layouts
+(src/ray-tracing/GpuModels.h:26-63).  There is no CPU fallback: if
+librtamd.so is missing or the GPU is absent the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "librtamd.so")
+
+RT_OK, RT_ERR_INVALID, RT_ERR_HIP, RT_ERR_NOMEM, RT_ERR_UNSUPPORTED, RT_ERR_BUSY, RT_ERR_CANCELLED, RT_ERR_NOT_FOUND = range(8)
+RT_MAT_LIGHT, RT_MAT_LAMBERTIAN, RT_MAT_METAL, RT_MAT_GLASS = range(4)
+TRAVERSE_AUTO, TRAVERSE_PACKED, TRAVERSE_REFERENCE = range(3)
+
+# --------------------------------------------------------------- records
+TRI_DT = np.dtype({"names": ["v0", "v1", "v2", "materialIndex"],
+                   "formats": [("<f4", 3), ("<f4", 3), ("<f4", 3), "<u4"],
+                   "offsets": [0, 16, 32, 44], "itemsize": 48})
+MAT_DT = np.dtype({"names": ["type", "albedo"], "formats": ["<u4", ("<f4", 3)],
+                   "offsets": [0, 16], "itemsize": 32})
+NODE_DT = np.dtype({"names": ["min", "max", "left", "right", "object"],
+                    "formats": [("<f4", 3), ("<f4", 3), "<i4", "<i4", "<i4"],
+                    "offsets": [0, 16, 28, 32, 36], "itemsize": 48})
+LIGHT_DT = np.dtype({"names": ["triangleIndex", "area"], "formats": ["<u4", "<f4"],
+                     "offsets": [0, 4], "itemsize": 8})
+SPHERE_DT = np.dtype({"names": ["s", "materialIndex"], "formats": [("<f4", 4), "<u4"],
+                      "offsets": [0, 16], "itemsize": 32})
+RAY_DT = np.dtype({"names": ["o", "tmin", "d", "tmax"],
+                   "formats": [("<f4", 3), "<f4", ("<f4", 3), "<f4"],
+                   "offsets": [0, 12, 16, 28], "itemsize": 32})
+HIT_DT = np.dtype({"names": ["id", "t", "u", "v"], "formats": ["<i4", "<f4", "<f4", "<f4"],
+                   "offsets": [0, 4, 8, 12], "itemsize": 16})
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("triangles", ctypes.c_void_p), ("num_triangles", ctypes.c_uint32),
+                ("materials", ctypes.c_void_p), ("num_materials", ctypes.c_uint32),
+                ("nodes", ctypes.c_void_p), ("num_nodes", ctypes.c_uint32),
+                ("lights", ctypes.c_void_p), ("num_lights", ctypes.c_uint32),
+                ("spheres", ctypes.c_void_p), ("num_spheres", ctypes.c_uint32)]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_uint32), ("num_nodes", ctypes.c_uint32),
+                ("num_lights", ctypes.c_uint32), ("num_spheres", ctypes.c_uint32),
+                ("num_materials", ctypes.c_uint32), ("tree_height", ctypes.c_uint32),
+                ("reachable_leaves", ctypes.c_uint32), ("enclosing", ctypes.c_uint32),
+                ("fast_path", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64)]
+
+
+class QueryParams(ctypes.Structure):
+    _fields_ = [("traversal", ctypes.c_uint32), ("max_stack", ctypes.c_uint32),
+                ("spheres", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("vfov_deg", ctypes.c_float), ("bounces", ctypes.c_uint32),
+                ("max_stack", ctypes.c_uint32), ("spheres", ctypes.c_uint32),
+                ("traversal", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class Ubo(ctypes.Structure):
+    _fields_ = [("camPos", ctypes.c_float * 3), ("time", ctypes.c_float),
+                ("currentSample", ctypes.c_uint32), ("numTriangles", ctypes.c_uint32),
+                ("numLights", ctypes.c_uint32), ("numSpheres", ctypes.c_uint32)]
+
+
+class WScene(ctypes.Structure):
+    _fields_ = [("triangles", ctypes.c_void_p), ("num_triangles", ctypes.c_uint32),
+                ("materials", ctypes.c_void_p), ("num_materials", ctypes.c_uint32),
+                ("nodes", ctypes.c_void_p), ("num_nodes", ctypes.c_uint32),
+                ("lights", ctypes.c_void_p), ("num_lights", ctypes.c_uint32),
+                ("spheres", ctypes.c_void_p), ("num_spheres", ctypes.c_uint32)]
+
+
+class RtError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+_lib = None
+
+# Every symbol include/rtamd.h and include/rtamd_workloads.h declare.
+EXPORTS = [
+    "rt_create", "rt_destroy", "rt_last_error", "rt_status_string", "rt_scene_upload", "rt_scene_info_get",
+    "rt_scene_check",
+    "rt_scene_release", "rt_closest_hit", "rt_closest_hit_stats", "rt_render", "rt_render_frames",
+    "rt_render_last_rays", "rt_bvh_build", "rt_bvh_build_seeded", "rt_scene_lights",
+    "rtw_scene_soup", "rtw_scene_icosphere", "rtw_scene_clusters", "rtw_scene_free", "rtw_rays_camera_tiles",
+    "rtw_rays_shell", "rtw_rays_box", "rtw_log_uniform",
+]
+
+
+def lib():
+    """Loads librtamd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run vulkan-compute-ray-tracing_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    P, u32, i32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64
+    sig = {
+        "rt_create": (i32, [ctypes.POINTER(P), i32]),
+        "rt_destroy": (i32, [P]),
+        "rt_last_error": (ctypes.c_char_p, [P]),
+        "rt_status_string": (ctypes.c_char_p, [i32]),
+        "rt_scene_upload": (i32, [P, ctypes.POINTER(SceneDesc), ctypes.POINTER(P)]),
+        "rt_scene_info_get": (i32, [P, ctypes.POINTER(SceneInfo)]),
+        "rt_scene_release": (i32, [P, P]),
+        "rt_scene_check": (i32, [ctypes.POINTER(SceneDesc), ctypes.POINTER(SceneInfo), ctypes.c_char_p, ctypes.c_size_t]),
+        "rt_closest_hit": (i32, [P, P, P, u32, P, ctypes.POINTER(QueryParams), P]),
+        "rt_closest_hit_stats": (i32, [P, P, P, u32, P, P, P, P]),
+        "rt_render": (i32, [P, P, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), P, P, u32, u32, P]),
+        "rt_render_frames": (i32, [P, P, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), u32, P, P, u32, u32, P]),
+        "rt_render_last_rays": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+        "rt_bvh_build": (i32, [P, u32, P]),
+        "rt_bvh_build_seeded": (i32, [P, u32, u32, u32, P]),
+        "rt_scene_lights": (i32, [P, u32, P, u32, P, u32, ctypes.POINTER(u32)]),
+        "rtw_scene_soup": (i32, [u32, u64, u32, ctypes.POINTER(WScene)]),
+        "rtw_scene_icosphere": (i32, [u32, u64, ctypes.c_float, ctypes.c_float, u32, ctypes.POINTER(WScene)]),
+        "rtw_scene_clusters": (i32, [u32, u64, u32, ctypes.POINTER(WScene)]),
+        "rtw_scene_free": (None, [ctypes.POINTER(WScene)]),
+        "rtw_rays_camera_tiles": (i32, [P, ctypes.c_float, u32, u32, u32, u32, u32, P]),
+        "rtw_rays_shell": (i32, [u32, u64, ctypes.c_float, ctypes.c_float, P]),
+        "rtw_rays_box": (i32, [u32, u64, ctypes.c_float, P]),
+        "rtw_log_uniform": (u32, [u64, u32, u32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _np_ptr(a):
+    return a.ctypes.data if a is not None and len(a) else None
+
+
+def _check(rc: int, ctx=None, what: str = ""):
+    if rc != RT_OK:
+        msg = lib().rt_last_error(ctx).decode() if ctx else lib().rt_status_string(rc).decode()
+        raise RtError(rc, f"{what}: {msg}" if what else msg)
+
+
+# ------------------------------------------------------------ host side
+class HostScene:
+    """Scene arrays in the reference's layouts (numpy)."""
+
+    def __init__(self, triangles, materials, nodes, lights=None, spheres=None):
+        self.triangles = np.ascontiguousarray(triangles, dtype=TRI_DT)
+        self.materials = np.ascontiguousarray(materials, dtype=MAT_DT)
+        self.nodes = np.ascontiguousarray(nodes, dtype=NODE_DT)
+        self.lights = np.ascontiguousarray(lights if lights is not None else np.zeros(0, LIGHT_DT), dtype=LIGHT_DT)
+        self.spheres = np.ascontiguousarray(spheres if spheres is not None else np.zeros(0, SPHERE_DT), dtype=SPHERE_DT)
+
+    def desc(self) -> SceneDesc:
+        return SceneDesc(_np_ptr(self.triangles), len(self.triangles), _np_ptr(self.materials), len(self.materials),
+                         _np_ptr(self.nodes), len(self.nodes), _np_ptr(self.lights), len(self.lights),
+                         _np_ptr(self.spheres), len(self.spheres))
+
+
+def _take(ws: WScene) -> HostScene:
+    def arr(ptr, n, dt):
+        if n == 0:
+            return np.zeros(0, dt)
+        buf = (ctypes.c_char * (n * dt.itemsize)).from_address(ptr)
+        # a bytearray keeps every byte (padding included) and stays writable
+        return np.frombuffer(bytearray(buf), dtype=dt)
+    try:
+        return HostScene(arr(ws.triangles, ws.num_triangles, TRI_DT), arr(ws.materials, ws.num_materials, MAT_DT),
+                         arr(ws.nodes, ws.num_nodes, NODE_DT), arr(ws.lights, ws.num_lights, LIGHT_DT),
+                         arr(ws.spheres, ws.num_spheres, SPHERE_DT))
+    finally:
+        lib().rtw_scene_free(ctypes.byref(ws))
+
+
+def scene_soup(n: int, seed: int, threads: int = 8) -> HostScene:
+    ws = WScene()
+    _check(lib().rtw_scene_soup(n, seed, threads, ctypes.byref(ws)), what="rtw_scene_soup")
+    return _take(ws)
+
+
+def scene_icosphere(level: int, seed: int, amp: float = 0.1, freq: float = 4.0, threads: int = 8) -> HostScene:
+    ws = WScene()
+    _check(lib().rtw_scene_icosphere(level, seed, amp, freq, threads, ctypes.byref(ws)), what="rtw_scene_icosphere")
+    return _take(ws)
+
+
+def scene_clusters(n: int, seed: int, threads: int = 8) -> HostScene:
+    ws = WScene()
+    _check(lib().rtw_scene_clusters(n, seed, threads, ctypes.byref(ws)), what="rtw_scene_clusters")
+    return _take(ws)
+
+
+def rays_camera_tiles(cam_pos, vfov, w, h, tile, first, count) -> np.ndarray:
+    out = np.zeros(count, RAY_DT)
+    cam = (ctypes.c_float * 3)(*[float(v) for v in cam_pos])
+    _check(lib().rtw_rays_camera_tiles(cam, vfov, w, h, tile, first, count, _np_ptr(out)), what="rtw_rays_camera_tiles")
+    return out
+
+
+def rays_shell(n, seed, radius=3.0, half=0.5) -> np.ndarray:
+    out = np.zeros(n, RAY_DT)
+    _check(lib().rtw_rays_shell(n, seed, radius, half, _np_ptr(out)), what="rtw_rays_shell")
+    return out
+
+
+def rays_box(n, seed, extent=4.0) -> np.ndarray:
+    out = np.zeros(n, RAY_DT)
+    _check(lib().rtw_rays_box(n, seed, extent, _np_ptr(out)), what="rtw_rays_box")
+    return out
+
+
+def log_uniform(seed: int, lo: int, hi: int) -> int:
+    return int(lib().rtw_log_uniform(seed, lo, hi))
+
+
+def bvh_build(triangles: np.ndarray, seed: int | None = None, threads: int = 1) -> np.ndarray:
+    """Bvh::createBvh.  seed=None consumes this process's rand()."""
+    tris = np.ascontiguousarray(triangles, TRI_DT)
+    out = np.zeros(max(0, 2 * len(tris) - 1), NODE_DT)
+    if seed is None:
+        rc = lib().rt_bvh_build(_np_ptr(tris), len(tris), _np_ptr(out))
+    else:
+        rc = lib().rt_bvh_build_seeded(_np_ptr(tris), len(tris), seed, threads, _np_ptr(out))
+    _check(rc, what="rt_bvh_build")
+    return out
+
+
+def scene_check(scene: HostScene):
+    """Validates a scene without a device; returns (status, SceneInfo, reason)."""
+    info = SceneInfo()
+    buf = ctypes.create_string_buffer(512)
+    d = scene.desc()
+    rc = lib().rt_scene_check(ctypes.byref(d), ctypes.byref(info), buf, 512)
+    return rc, info, buf.value.decode()
+
+
+def scene_lights(triangles: np.ndarray, materials: np.ndarray) -> np.ndarray:
+    tris = np.ascontiguousarray(triangles, TRI_DT)
+    mats = np.ascontiguousarray(materials, MAT_DT)
+    cnt = ctypes.c_uint32(0)
+    _check(lib().rt_scene_lights(_np_ptr(tris), len(tris), _np_ptr(mats), len(mats), None, 0, ctypes.byref(cnt)),
+           what="rt_scene_lights")
+    out = np.zeros(cnt.value, LIGHT_DT)
+    _check(lib().rt_scene_lights(_np_ptr(tris), len(tris), _np_ptr(mats), len(mats), _np_ptr(out), cnt.value,
+                                 ctypes.byref(cnt)), what="rt_scene_lights")
+    return out
+
+
+# ------------------------------------------------------------ device side
+def _dev_ptr(t) -> int:
+    return int(t.data_ptr())
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return int(stream.cuda_stream) or None
+
+
+def rays_to_device(rays: np.ndarray, device="cuda"):
+    import torch
+    r = np.ascontiguousarray(rays, RAY_DT)
+    return torch.from_numpy(r.view(np.float32).reshape(len(r), 8).copy()).to(device)
+
+
+def hits_from_device(t) -> np.ndarray:
+    a = t.detach().cpu().contiguous().numpy()
+    return a.view(np.uint8).reshape(-1).view(HIT_DT).copy()
+
+
+class Context:
+    """An rt_ctx on one device."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        rc = lib().rt_create(ctypes.byref(self._h), device)
+        if rc != RT_OK:
+            raise RtError(rc, f"rt_create(device={device}) failed: {lib().rt_status_string(rc).decode()}")
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().rt_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc, what=""):
+        _check(rc, self._h, what)
+
+    def upload(self, scene: HostScene) -> "Scene":
+        h = ctypes.c_void_p()
+        d = scene.desc()
+        self.check(lib().rt_scene_upload(self._h, ctypes.byref(d), ctypes.byref(h)), "rt_scene_upload")
+        return Scene(self, h)
+
+    def closest_hit(self, scene: "Scene", rays_dev, hits_dev=None, traversal=TRAVERSE_AUTO, max_stack=0, spheres=0,
+                    stream=None, n=None):
+        import torch
+        n = rays_dev.shape[0] if n is None else n
+        if hits_dev is None:
+            hits_dev = torch.empty((n, 4), dtype=torch.float32, device=rays_dev.device)
+        p = QueryParams(traversal, max_stack, spheres, 0)
+        self.check(lib().rt_closest_hit(self._h, scene.handle, _dev_ptr(rays_dev), n, _dev_ptr(hits_dev),
+                                        ctypes.byref(p), _stream_ptr(stream)), "rt_closest_hit")
+        return hits_dev
+
+    def closest_hit_stats(self, scene: "Scene", rays_dev, stream=None):
+        import torch
+        n = rays_dev.shape[0]
+        hits = torch.empty((n, 4), dtype=torch.float32, device=rays_dev.device)
+        nv = torch.empty(n, dtype=torch.int32, device=rays_dev.device)
+        tt = torch.empty(n, dtype=torch.int32, device=rays_dev.device)
+        self.check(lib().rt_closest_hit_stats(self._h, scene.handle, _dev_ptr(rays_dev), n, _dev_ptr(hits),
+                                              _dev_ptr(nv), _dev_ptr(tt), _stream_ptr(stream)), "rt_closest_hit_stats")
+        return hits, nv, tt
+
+    def render(self, scene: "Scene", cam_pos, w, h, current_sample=0, vfov=30.0, bounces=2, max_stack=0, spheres=0,
+               traversal=TRAVERSE_AUTO, target=None, accum=None, stream=None):
+        """One frame (rt_render); returns the RGBA8 target as a [h, w, 4] uint8 tensor."""
+        import torch
+        dev = f"cuda:{self.device}"
+        if target is None:
+            target = torch.zeros((h, w, 4), dtype=torch.uint8, device=dev)
+        if accum is None:
+            accum = torch.zeros((h, w, 4), dtype=torch.uint8, device=dev)
+        ubo = make_ubo(cam_pos, current_sample, scene.info)
+        prm = RenderParams(vfov, bounces, max_stack, spheres, traversal)
+        self.check(lib().rt_render(self._h, scene.handle, ctypes.byref(ubo), ctypes.byref(prm), _dev_ptr(target),
+                                   _dev_ptr(accum), w, h, _stream_ptr(stream)), "rt_render")
+        return target
+
+    def render_frames(self, scene: "Scene", cam_pos, w, h, frames, first_sample=0, vfov=30.0, bounces=2, max_stack=0,
+                      spheres=0, traversal=TRAVERSE_AUTO, target=None, accum=None, stream=None):
+        import torch
+        dev = f"cuda:{self.device}"
+        if target is None:
+            target = torch.zeros((h, w, 4), dtype=torch.uint8, device=dev)
+        if accum is None:
+            accum = torch.zeros((h, w, 4), dtype=torch.uint8, device=dev)
+        ubo = make_ubo(cam_pos, first_sample, scene.info)
+        prm = RenderParams(vfov, bounces, max_stack, spheres, traversal)
+        self.check(lib().rt_render_frames(self._h, scene.handle, ctypes.byref(ubo), ctypes.byref(prm), frames,
+                                          _dev_ptr(target), _dev_ptr(accum), w, h, _stream_ptr(stream)),
+                   "rt_render_frames")
+        return target, accum
+
+    def last_rays(self):
+        p, s = ctypes.c_uint64(), ctypes.c_uint64()
+        self.check(lib().rt_render_last_rays(self._h, ctypes.byref(p), ctypes.byref(s)), "rt_render_last_rays")
+        return p.value, s.value
+
+
+class Scene:
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.handle = handle
+        info = SceneInfo()
+        _check(lib().rt_scene_info_get(handle, ctypes.byref(info)), what="rt_scene_info_get")
+        self.info = info
+
+    def release(self):
+        if self.handle:
+            self.ctx.check(lib().rt_scene_release(self.ctx.handle, self.handle), "rt_scene_release")
+            self.handle = None
+
+
+def make_ubo(cam_pos, current_sample=0, info=None, time=0.0) -> Ubo:
+    u = Ubo()
+    for k in range(3):
+        u.camPos[k] = float(cam_pos[k])
+    u.time = time
+    u.currentSample = current_sample
+    if info is not None:
+        u.numTriangles, u.numLights, u.numSpheres = info.num_triangles, info.num_lights, info.num_spheres
+    return u
