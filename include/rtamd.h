@@ -89,7 +89,8 @@ typedef struct rt_ray {
 
 /* Closest hit: id = triangle index (or num_triangles + sphere index when
  * spheres are enabled), -1 = miss.  t, u, v as triIntersect returns them
- * (ray-trace-compute.comp:157-173): u weights v1, v weights v2. */
+ * (ray-trace-compute.comp:157-173): u weights v1, v weights v2.  A sphere
+ * hit reports u = backFaceInt (1 when the far root was taken), v = 0. */
 typedef struct rt_hit {
     int32_t id; float t; float u; float v;
 } rt_hit;                                        /* 16 bytes */

@@ -312,6 +312,16 @@ static int hit_bvh_ref(const orc_scene* sc, ray_t r, float t_min, float t_max, u
  * closest hit.  Its counts are the canonical per-ray work of the packed GPU
  * traversal: `nodes` counts internal nodes expanded (one two-box record
  * each), `tris` counts triangle tests. */
+/* Reciprocal used by the box tests: a zero (or tiny) direction component is
+ * replaced by +-2^-80 so every slab bound stays finite; the ray it describes
+ * is within 1e-20 of the true one over t <= 1e4, far inside the 1e-4 box
+ * padding, so the test stays conservative. */
+static inline float safe_inv(float d) {
+    const float tiny = 0x1p-80f;
+    float a = fabsf(d) < tiny ? copysignf(tiny, d) : d;
+    return 1.0f / a;
+}
+
 static int box_interval(v3 o, v3 invd, const o_node* nd, float tmin, float tmax, float* tn) {
     float tx0 = (nd->mn[0] - o.x) * invd.x, tx1 = (nd->mx[0] - o.x) * invd.x;
     float ty0 = (nd->mn[1] - o.y) * invd.y, ty1 = (nd->mx[1] - o.y) * invd.y;
@@ -328,7 +338,7 @@ static int hit_bvh_ordered(const orc_scene* sc, ray_t r, float t_min, float t_ma
                            hitrec* rec, int* id, float* uv, trav_stats* st) {
     int hit_anything = 0;
     float closest = t_max;
-    v3 invd = V(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    v3 invd = V(safe_inv(r.d.x), safe_inv(r.d.y), safe_inv(r.d.z));
     int stack[ORC_ORDERED_STACK];
     int sp = 0;
     int cur = 0;
@@ -403,7 +413,8 @@ static int closest_hit(const orc_scene* sc, ray_t r, float tmin, float tmax, int
             hitrec tmp;
             if (hit_sphere(sc, (int)j, r, tmin, closest, &tmp)) {
                 hit = 1; closest = tmp.t; *rec = tmp;
-                *id = (int)(sc->ntri + j); uv[0] = uv[1] = 0.0f;
+                /* a sphere hit reports u = backFaceInt, v = 0 */
+                *id = (int)(sc->ntri + j); uv[0] = (float)tmp.back; uv[1] = 0.0f;
             }
         }
     }

@@ -68,7 +68,8 @@ extern "C" int rt_create(rt_ctx** out, int device) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return RT_ERR_HIP; }
     c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (hipMalloc((void**)&c->d_counters, 64 * sizeof(uint32_t)) != hipSuccess) { delete c; return RT_ERR_NOMEM; }
-    if (hipMemset(c->d_counters, 0, 64 * sizeof(uint32_t)) != hipSuccess) {
+    if (hipMemset(c->d_counters, 0, 64 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&c->d_cursors, kCursorSlots * sizeof(uint32_t)) != hipSuccess) {
         (void)hipFree(c->d_counters);
         delete c;
         return RT_ERR_HIP;
@@ -84,6 +85,7 @@ extern "C" int rt_destroy(rt_ctx* ctx) {
     for (int i = 0; i < 3; ++i)
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_cursors) (void)hipFree(ctx->d_cursors);
     delete ctx;
     return RT_OK;
 }

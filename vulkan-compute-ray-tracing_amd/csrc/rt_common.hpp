@@ -104,6 +104,9 @@ struct rt_ctx {
     void* ws[3] = {nullptr, nullptr, nullptr};
     size_t ws_bytes[3] = {0, 0, 0};
     uint32_t* d_counters = nullptr;   // small zero-filled counter block
+    uint32_t* d_cursors = nullptr;    // ring of persistent-kernel ray cursors
+    uint32_t cursor_next = 0;
+    uint32_t tune_refill = 0;         // lanes idle before a wave refetches (0 = default)
     uint64_t last_primary = 0, last_secondary = 0;
 };
 
@@ -112,6 +115,7 @@ namespace rtamd {
 int set_error(rt_ctx* ctx, int code, const std::string& msg);
 int hip_error(rt_ctx* ctx, hipError_t e, const char* what);
 enum { kWsRender = 0, kWsStack = 1, kWsStats = 2 };
+constexpr uint32_t kCursorSlots = 1024;
 int ensure_workspace(rt_ctx* ctx, int slot, size_t bytes);
 
 #define RT_HIP(ctx, call)                                              \

@@ -96,25 +96,83 @@ __device__ __forceinline__ bool sphere_test(const rt_sphere& s, const f3& o, con
     return true;
 }
 
+// Reciprocal for the box tests: a zero (or tiny) direction component
+// becomes +-2^-80 so every slab bound stays finite (an infinite bound would
+// make the slack term infinite and keep every box).  The ray this describes
+// departs from the true one by < 1e-20 over t <= 1e4, far inside the boxes'
+// padding, so the test stays conservative.
+__device__ __forceinline__ float safe_inv(float d) {
+    const float tiny = 0x1p-80f;
+    float a = fabsf(d) < tiny ? copysignf(tiny, d) : d;
+    return 1.0f / a;
+}
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Persistent traversal with per-lane ray fetch.  Each lane owns one ray at a
+// time; when at least `refill` lanes of the wave are idle, the wave takes
+// that many ray indices from the global cursor with one atomic and hands
+// them out by ballot prefix count, so a long ray never holds its wave's
+// other lanes idle.  One loop iteration reads one node record or one
+// triangle record per busy lane.
 template <int BLOCK, int STACK, bool STATS>
 __global__ __launch_bounds__(BLOCK) void k_trace_packed(
     const PNode* __restrict__ nodes, const PTri* __restrict__ tris, const int32_t* __restrict__ tri_id,
     int32_t root, const rt_ray* __restrict__ rays, uint32_t n_host, const uint32_t* __restrict__ n_dev,
     rt_hit* __restrict__ hits, const rt_sphere* __restrict__ spheres, uint32_t nspheres, uint32_t ntri,
-    uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests) {
+    uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests, uint32_t* __restrict__ cursor,
+    uint32_t refill) {
     __shared__ int32_t stk[STACK * BLOCK];
     const uint32_t n = n_dev ? *n_dev : n_host;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t i = blockIdx.x * BLOCK + tid; i < n; i += gridDim.x * BLOCK) {
-        RayIn r = load_ray(rays, i);
-        const f3 inv{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
-        float best_t = r.tmax, bu = 0.0f, bv = 0.0f;
-        int32_t best = -1;
-        uint32_t nv = 0, nt = 0;
-        int sp = 0;
-        int32_t cur = root;
-        if (cur != kEmptyRoot) {
-            for (;;) {
+    const uint32_t lane = lane_id();
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+
+    bool busy = false;
+    bool drained = false;           // wave-uniform: the cursor passed n
+    uint32_t ray_i = 0;
+    RayIn r{};
+    f3 inv{0.0f, 0.0f, 0.0f};
+    float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
+    int32_t best = -1;
+    uint32_t nv = 0, nt = 0;
+    int sp = 0;
+    int32_t cur = 0;
+
+    for (;;) {
+        // ---- refill idle lanes
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!drained && n_idle >= refill) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(idle);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(cursor, n_idle);
+            base = __shfl(base, (int)leader);
+            if (base + n_idle >= n) drained = true;
+            if (!busy) {
+                uint32_t mine = base + (uint32_t)__popcll(idle & lanes_below);
+                if (mine < n) {
+                    ray_i = mine;
+                    r = load_ray(rays, mine);
+                    inv = f3{safe_inv(r.d.x), safe_inv(r.d.y), safe_inv(r.d.z)};
+                    best_t = r.tmax; bu = 0.0f; bv = 0.0f; best = -1;
+                    nv = 0; nt = 0; sp = 0;
+                    cur = root;
+                    busy = true;
+                    if (root == kEmptyRoot) cur = 0, sp = -1;   // finishes on the first step
+                }
+            }
+        }
+        if (__ballot(busy) == 0) {
+            if (drained) break;
+            continue;
+        }
+        // ---- one traversal step per busy lane
+        if (busy) {
+            bool pop = true;
+            if (sp >= 0) {
                 if (cur >= 0) {
                     const float4* np = reinterpret_cast<const float4*>(nodes + cur);
                     float4 a = np[0], b = np[1], c = np[2];
@@ -125,15 +183,14 @@ __global__ __launch_bounds__(BLOCK) void k_trace_packed(
                     bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, r.o, inv, r.tmin, best_t, tn1);
                     if (h0 && h1) {
                         bool swap = tn1 < tn0;
-                        int32_t nearc = swap ? dd.y : dd.x;
-                        int32_t farc = swap ? dd.x : dd.y;
-                        stk[sp * BLOCK + tid] = farc;
+                        stk[sp * BLOCK + tid] = swap ? dd.x : dd.y;
                         sp++;
-                        cur = nearc;
-                        continue;
+                        cur = swap ? dd.y : dd.x;
+                        pop = false;
+                    } else if (h0 || h1) {
+                        cur = h0 ? dd.x : dd.y;
+                        pop = false;
                     }
-                    if (h0) { cur = dd.x; continue; }
-                    if (h1) { cur = dd.y; continue; }
                 } else {
                     const int32_t slot = ~cur;
                     const float4* tp = reinterpret_cast<const float4*>(tris + slot);
@@ -144,24 +201,31 @@ __global__ __launch_bounds__(BLOCK) void k_trace_packed(
                         best_t = t; bu = u; bv = v; best = slot;
                     }
                 }
-                if (sp == 0) break;
-                sp--;
-                cur = stk[sp * BLOCK + tid];
             }
-        }
-        int32_t id = best >= 0 ? tri_id[best] : -1;
-        if (nspheres) {
-            float closest = best >= 0 ? best_t : r.tmax;
-            for (uint32_t j = 0; j < nspheres; ++j) {
-                float root_t; int back;
-                if (sphere_test(spheres[j], r.o, r.d, r.tmin, closest, root_t, back)) {
-                    closest = root_t; best_t = root_t; id = (int32_t)(ntri + j); bu = (float)back; bv = 0.0f;
+            if (pop) {
+                if (sp > 0) {
+                    sp--;
+                    cur = stk[sp * BLOCK + tid];
+                } else {
+                    // ---- ray finished: spheres, then the result
+                    int32_t id = best >= 0 ? tri_id[best] : -1;
+                    if (nspheres) {
+                        float closest = best >= 0 ? best_t : r.tmax;
+                        for (uint32_t j = 0; j < nspheres; ++j) {
+                            float root_t; int back;
+                            if (sphere_test(spheres[j], r.o, r.d, r.tmin, closest, root_t, back)) {
+                                closest = root_t; best_t = root_t; id = (int32_t)(ntri + j);
+                                bu = (float)back; bv = 0.0f;
+                            }
+                        }
+                    }
+                    if (id >= 0) store_hit(hits, ray_i, id, best_t, bu, bv);
+                    else store_hit(hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
+                    if (STATS) { node_visits[ray_i] = nv; tri_tests[ray_i] = nt; }
+                    busy = false;
                 }
             }
         }
-        if (id >= 0) store_hit(hits, i, id, best_t, bu, bv);
-        else store_hit(hits, i, -1, 0.0f, 0.0f, 0.0f);
-        if (STATS) { node_visits[i] = nv; tri_tests[i] = nt; }
     }
 }
 
@@ -249,17 +313,38 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t value, uint32_t count) {
 
 constexpr int kBlock = 256;
 
+// Ray cursors for persistent launches: a ring of counters so launches on
+// different streams never share one; each is zeroed by a fill kernel on the
+// launch's own stream right before use.
+uint32_t* next_cursor(rt_ctx* ctx) {
+    uint32_t k = ctx->cursor_next++ % kCursorSlots;
+    return ctx->d_cursors + k;
+}
+
 template <int STACK>
-int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, uint32_t blocks, bool stats) {
+int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
     const rt_scene* sc = a.scene;
-    if (stats)
-        hipLaunchKernelGGL((k_trace_packed<kBlock, STACK, true>), dim3(blocks), dim3(kBlock), 0, s, sc->d_pnodes,
-                           sc->d_ptris, sc->d_ptri_id, sc->root_ref, a.rays, a.n, a.n_dev, a.hits, sc->d_spheres,
-                           a.spheres ? sc->info.num_spheres : 0u, sc->info.num_triangles, a.node_visits, a.tri_tests);
-    else
-        hipLaunchKernelGGL((k_trace_packed<kBlock, STACK, false>), dim3(blocks), dim3(kBlock), 0, s, sc->d_pnodes,
-                           sc->d_ptris, sc->d_ptri_id, sc->root_ref, a.rays, a.n, a.n_dev, a.hits, sc->d_spheres,
-                           a.spheres ? sc->info.num_spheres : 0u, sc->info.num_triangles, a.node_visits, a.tri_tests);
+    auto kern = stats ? k_trace_packed<kBlock, STACK, true> : k_trace_packed<kBlock, STACK, false>;
+    int per_cu = 0;
+    RT_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
+    if (per_cu < 1) per_cu = 1;
+    uint32_t blocks = (uint32_t)ctx->num_cus * (uint32_t)per_cu;
+    if (!a.n_dev) {
+        // no more waves than rays
+        uint32_t need = (a.n + kBlock - 1) / kBlock;
+        if (need < blocks) blocks = need;
+        if (blocks == 0) return RT_OK;
+    }
+    uint32_t* cursor = next_cursor(ctx);
+    int rc = launch_fill_u32(ctx, cursor, 0u, 1u, s);
+    if (rc) return rc;
+    // 1..64 idle lanes trigger a refetch (64: only when the whole wave is idle)
+    uint32_t refill = ctx->tune_refill ? ctx->tune_refill : 16u;
+    if (refill > 64u) refill = 64u;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, sc->d_pnodes, sc->d_ptris, sc->d_ptri_id,
+                       sc->root_ref, a.rays, a.n, a.n_dev, a.hits, sc->d_spheres,
+                       a.spheres ? sc->info.num_spheres : 0u, sc->info.num_triangles, a.node_visits, a.tri_tests,
+                       cursor, refill);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;
 }
@@ -270,20 +355,12 @@ int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s) {
     const rt_scene* sc = a.scene;
     if (!sc->fast) return set_error(ctx, RT_ERR_UNSUPPORTED, "packed traversal needs a tree that encloses its triangles");
     const bool stats = a.node_visits && a.tri_tests;
-    // grid: enough blocks to fill every CU several times; a grid-stride loop
-    // covers the rest (and device-side counts from the render queues).
-    uint32_t blocks = (uint32_t)ctx->num_cus * 8u;
-    if (!a.n_dev) {
-        uint32_t need = (a.n + kBlock - 1) / kBlock;
-        if (need < blocks) blocks = need;
-        if (blocks == 0) return RT_OK;
-    }
     const uint32_t h = sc->packed_height;
-    if (h <= 16) return launch_packed_t<16>(ctx, a, s, blocks, stats);
-    if (h <= 24) return launch_packed_t<24>(ctx, a, s, blocks, stats);
-    if (h <= 32) return launch_packed_t<32>(ctx, a, s, blocks, stats);
-    if (h <= 48) return launch_packed_t<48>(ctx, a, s, blocks, stats);
-    if (h <= 64) return launch_packed_t<64>(ctx, a, s, blocks, stats);
+    if (h <= 16) return launch_packed_t<16>(ctx, a, s, stats);
+    if (h <= 24) return launch_packed_t<24>(ctx, a, s, stats);
+    if (h <= 32) return launch_packed_t<32>(ctx, a, s, stats);
+    if (h <= 48) return launch_packed_t<48>(ctx, a, s, stats);
+    if (h <= 64) return launch_packed_t<64>(ctx, a, s, stats);
     return set_error(ctx, RT_ERR_UNSUPPORTED, "packed tree deeper than 64 levels");
 }
 
