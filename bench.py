@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark of the ray-traversal hot path on one MI355X.
+
+Step = one rt_closest_hit over the whole ray batch (the plain kernel) on the
+clustered-soup scene: 4,194,304 triangles in 4096 Gaussian clusters (centres
+U[-4,4]^3, sigma 0.15, edges U[0.005,0.03]), BVH from the reference builder,
+and 4,194,304 incoherent rays (origins U[-4,4]^3, directions uniform on the
+sphere), all resident in HBM before the timed region.
+
+Prints one JSON line: throughput in Mrays/s (one ray = one hit_bvh
+traversal), the roofline of the traversal kernel (algorithmic bytes = node
+records x 64 B + triangle records x 48 B + ray 32 B + hit 16 B, counted per
+ray by an untimed pass of the counting kernel), the CPU baseline (the
+oracle's restatement of hit_bvh in the shader's visiting order, on a bounded
+sample of the same rays), and side measurements of the 1920x1080 x 4 spp,
+8-bounce render of the same scene.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "vulkan-compute-ray-tracing_amd")]
+
+METRIC = "incoherent Mrays/s + p50/p99 latency; batched/served vs plain kernel, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_packed_clusters4M_traffic.txt")
+
+
+def host_threads() -> int:
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def read_traffic():
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("bytes_per_launch="):
+                return float(line.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(hs, rays_np, seconds: float, threads: int):
+    import oracle as O
+    osc = O.Scene(hs.triangles, hs.materials, hs.nodes, hs.lights, hs.spheres)
+    m = 64 * threads
+    t0 = time.perf_counter()
+    O.closest_hit(osc, rays_np[:m], O.TRAV_REFERENCE, threads=threads)
+    dt = time.perf_counter() - t0
+    m = int(min(len(rays_np), max(m, m * seconds / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    O.closest_hit(osc, rays_np[:m], O.TRAV_REFERENCE, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(m / dt / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"first {m} of the {len(rays_np)} incoherent rays, oracle hit_bvh in the shader's "
+                      f"visiting order (line test, both children pushed), {threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tris", type=int, default=1 << 22)
+    ap.add_argument("--rays", type=int, default=1 << 22)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-render", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (rays shard by batch with no "
+              "exchange; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import rtamd
+
+    threads = host_threads()
+    ctx = rtamd.Context(0)
+    hs = rtamd.scene_clusters(args.tris, 3, threads=threads)
+    scene = ctx.upload(hs)
+    rays_np = rtamd.rays_box(args.rays, 4, 4.0)
+    rays = rtamd.rays_to_device(rays_np)
+    n = len(rays_np)
+    hits = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+
+    # untimed counting pass: per-ray node and triangle records read
+    _, nv, tt = ctx.closest_hit_stats(scene, rays)
+    torch.cuda.synchronize()
+    v_node = int(nv.to(torch.int64).sum().item())
+    v_tri = int(tt.to(torch.int64).sum().item())
+    bytes_per_launch = v_node * 64 + v_tri * 48 + n * (32 + 16)
+    del nv, tt
+
+    for _ in range(args.warmup):
+        ctx.closest_hit(scene, rays, hits)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        evs[k][0].record()
+        ctx.closest_hit(scene, rays, hits)
+        evs[k][1].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    kernel_ms = sum(a.elapsed_time(b) for a, b in evs) / args.steps
+    value = n / (ms_per_step / 1e3) / 1e6
+    achieved = bytes_per_launch / (kernel_ms / 1e3) / 1e9
+    traffic = read_traffic()
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"closest-hit query, clustered soup {len(hs.triangles)} triangles in 4096 Gaussian "
+                               f"clusters, {n} incoherent rays (origins U[-4,4]^3, uniform directions)",
+                   "triangles": len(hs.triangles), "rays": n, "traversal": "packed ordered, persistent"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_trace_packed", "kernel_ms": round(kernel_ms, 4),
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "visits": {"node_records_per_ray": round(v_node / n, 2), "tri_records_per_ray": round(v_tri / n, 3)},
+    }
+
+    if not args.no_render:
+        tgt = torch.zeros((1080, 1920, 4), dtype=torch.uint8, device="cuda")
+        acc = torch.zeros_like(tgt)
+        ctx.render_frames(scene, [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8, target=tgt, accum=acc)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ctx.render_frames(scene, [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8, target=tgt, accum=acc)
+        e1.record()
+        torch.cuda.synchronize()
+        rms = e0.elapsed_time(e1)
+        prim, sec = ctx.last_rays()
+        out["render"] = {"workload": "1920x1080, 4 spp (currentSample 0..3), 8 bounces, vfov 60, eye (0,0,10)",
+                         "ms": round(rms, 3), "rays": prim + sec, "secondary_rays": sec,
+                         "mrays_per_s": round((prim + sec) / (rms / 1e3) / 1e6, 2)}
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(hs, rays_np, args.cpu_seconds, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -134,6 +134,7 @@ struct TraceArgs {
     uint32_t spheres;
     uint32_t* node_visits;      // optional per-ray counters
     uint32_t* tri_tests;
+    bool render = false;        // a bounce of the render's wavefront loop
 };
 int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
 int launch_trace_reference(rt_ctx* ctx, const TraceArgs& a, uint32_t max_stack, hipStream_t s);

@@ -313,7 +313,7 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
 
     for (uint32_t b = 0; b < bounces; ++b) {
         const uint32_t* n_dev = b == 0 ? nullptr : counters + b;
-        TraceArgs ta{sc, rays[b & 1], P, n_dev, hits, prm.spheres, nullptr, nullptr};
+        TraceArgs ta{sc, rays[b & 1], P, n_dev, hits, prm.spheres, nullptr, nullptr, true};
         rc = use_packed ? launch_trace_packed(ctx, ta, s) : launch_trace_reference(ctx, ta, prm.max_stack, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_shade, dim3(grid_for(P, cu_cap)), dim3(256), 0, s, rays[b & 1], pids[b & 1], n_dev, P,

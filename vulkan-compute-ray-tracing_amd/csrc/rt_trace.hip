@@ -118,13 +118,12 @@ __device__ __forceinline__ uint32_t lane_id() {
 // other lanes idle.  One loop iteration reads one node record or one
 // triangle record per busy lane.
 template <int BLOCK, int STACK, bool STATS>
-__global__ __launch_bounds__(BLOCK) void k_trace_packed(
+__device__ __forceinline__ void trace_persistent(
     const PNode* __restrict__ nodes, const PTri* __restrict__ tris, const int32_t* __restrict__ tri_id,
     int32_t root, const rt_ray* __restrict__ rays, uint32_t n_host, const uint32_t* __restrict__ n_dev,
     rt_hit* __restrict__ hits, const rt_sphere* __restrict__ spheres, uint32_t nspheres, uint32_t ntri,
     uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests, uint32_t* __restrict__ cursor,
-    uint32_t refill) {
-    __shared__ int32_t stk[STACK * BLOCK];
+    uint32_t refill, int32_t* __restrict__ stk) {
     const uint32_t n = n_dev ? *n_dev : n_host;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
@@ -229,6 +228,29 @@ __global__ __launch_bounds__(BLOCK) void k_trace_packed(
     }
 }
 
+#define RT_TRACE_PARAMS                                                                                        \
+    const PNode *__restrict__ nodes, const PTri *__restrict__ tris, const int32_t *__restrict__ tri_id,        \
+        int32_t root, const rt_ray *__restrict__ rays, uint32_t n_host, const uint32_t *__restrict__ n_dev,     \
+        rt_hit *__restrict__ hits, const rt_sphere *__restrict__ spheres, uint32_t nspheres, uint32_t ntri,     \
+        uint32_t *__restrict__ node_visits, uint32_t *__restrict__ tri_tests, uint32_t *__restrict__ cursor,    \
+        uint32_t refill
+#define RT_TRACE_ARGS \
+    nodes, tris, tri_id, root, rays, n_host, n_dev, hits, spheres, nspheres, ntri, node_visits, tri_tests, cursor, refill
+
+// Closest-hit queries (rt_closest_hit, rt_closest_hit_stats).
+template <int BLOCK, int STACK, bool STATS>
+__global__ __launch_bounds__(BLOCK) void k_trace_packed(RT_TRACE_PARAMS) {
+    __shared__ int32_t stk[STACK * BLOCK];
+    trace_persistent<BLOCK, STACK, STATS>(RT_TRACE_ARGS, stk);
+}
+
+// The render's per-bounce traces over the live-ray queue (count on device).
+template <int BLOCK, int STACK>
+__global__ __launch_bounds__(BLOCK) void k_trace_render(RT_TRACE_PARAMS) {
+    __shared__ int32_t stk[STACK * BLOCK];
+    trace_persistent<BLOCK, STACK, false>(RT_TRACE_ARGS, stk);
+}
+
 // intersectAABB (ray-trace-compute.comp:250-258) with GLSL min/max.
 __device__ __forceinline__ void aabb_line(const rt_bvh_node& nd, const f3& o, const f3& d, float& tn, float& tf) {
     f3 tMin = (ld3(nd.min) - o) / d;
@@ -324,7 +346,9 @@ uint32_t* next_cursor(rt_ctx* ctx) {
 template <int STACK>
 int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
     const rt_scene* sc = a.scene;
-    auto kern = stats ? k_trace_packed<kBlock, STACK, true> : k_trace_packed<kBlock, STACK, false>;
+    auto kern = a.render ? k_trace_render<kBlock, STACK>
+                : stats    ? k_trace_packed<kBlock, STACK, true>
+                           : k_trace_packed<kBlock, STACK, false>;
     int per_cu = 0;
     RT_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
     if (per_cu < 1) per_cu = 1;
