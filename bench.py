@@ -9,8 +9,9 @@ sphere), all resident in HBM before the timed region.
 
 Prints one JSON line: throughput in Mrays/s (one ray = one hit_bvh
 traversal), the roofline of the traversal kernel (algorithmic bytes = node
-records x 64 B + triangle records x 48 B + ray 32 B + hit 16 B, counted per
-ray by an untimed pass of the counting kernel), the CPU baseline (the
+records x 64 B (4-wide; 32 B with --layout q16) + triangle records x 48 B + ray 32 B +
+hit 16 B, counted per ray by an untimed pass of the counting kernel), the
+CPU baseline (the
 oracle's restatement of hit_bvh in the shader's visiting order, on a bounded
 sample of the same rays), and side measurements of the 1920x1080 x 4 spp,
 8-bounce render of the same scene.
@@ -73,6 +74,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--layout", choices=["w4", "q16", "f32"], default="w4",
+                    help="node records: 4-wide 64-byte 16-bit (default), binary 32-byte 16-bit, binary 64-byte fp32")
+    ap.add_argument("--refill", type=int, default=0, help="idle lanes before a wave refetches (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard by batch with no "
@@ -84,6 +88,9 @@ def main():
 
     threads = host_threads()
     ctx = rtamd.Context(0)
+    layout = {"w4": rtamd.LAYOUT_W4, "q16": rtamd.LAYOUT_Q16, "f32": rtamd.LAYOUT_F32}[args.layout]
+    node_bytes = 32 if layout == rtamd.LAYOUT_Q16 else 64
+    ctx.set_tuning(node_layout=layout, refill=args.refill)
     hs = rtamd.scene_clusters(args.tris, 3, threads=threads)
     scene = ctx.upload(hs)
     rays_np = rtamd.rays_box(args.rays, 4, 4.0)
@@ -96,7 +103,7 @@ def main():
     torch.cuda.synchronize()
     v_node = int(nv.to(torch.int64).sum().item())
     v_tri = int(tt.to(torch.int64).sum().item())
-    bytes_per_launch = v_node * 64 + v_tri * 48 + n * (32 + 16)
+    bytes_per_launch = v_node * node_bytes + v_tri * 48 + n * (32 + 16)
     del nv, tt
 
     for _ in range(args.warmup):
@@ -122,7 +129,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"closest-hit query, clustered soup {len(hs.triangles)} triangles in 4096 Gaussian "
                                f"clusters, {n} incoherent rays (origins U[-4,4]^3, uniform directions)",
-                   "triangles": len(hs.triangles), "rays": n, "traversal": "packed ordered, persistent"},
+                   "triangles": len(hs.triangles), "rays": n,
+                   "traversal": f"ordered, persistent, {args.layout} {node_bytes}-byte node records"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_trace_packed", "kernel_ms": round(kernel_ms, 4),

@@ -166,6 +166,21 @@ int rt_closest_hit_stats(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays_
                          uint32_t n, rt_hit* hits_dev, uint32_t* node_visits_dev,
                          uint32_t* tri_tests_dev, void* stream);
 
+/* ------------------------------------------------------------------ tuning */
+/* Launch parameters of the packed traversal (no counterpart in the
+ * reference, whose only knob is the compile-time MAX_STACK_DEPTH).  Zero
+ * fields keep the library default.  They never change results. */
+enum { RT_LAYOUT_AUTO = 0, RT_LAYOUT_F32 = 1, RT_LAYOUT_Q16 = 2, RT_LAYOUT_W4 = 3 };
+typedef struct rt_tuning {
+    uint32_t node_layout;   /* RT_LAYOUT_*: binary 64-byte fp32 or 32-byte 16-bit
+                               nodes, or 4-wide 64-byte 16-bit nodes            */
+    uint32_t refill;        /* idle lanes (1..64) that make a wave fetch rays     */
+    uint32_t blocks_per_cu; /* persistent blocks per CU (0 = occupancy limit)     */
+    uint32_t reserved[5];
+} rt_tuning;
+int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t);
+int rt_tuning_get(const rt_ctx* ctx, rt_tuning* t);
+
 /* ------------------------------------------------------------------ render */
 typedef struct rt_render_params {
     float    vfov_deg;    /* shader hard-codes 30 (ray-trace-compute.comp:358) */

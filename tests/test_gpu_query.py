@@ -64,19 +64,48 @@ def soup(rt, gpu_ctx):
 
 
 def test_soup_packed_matches_oracle_ordered(rt, gpu_ctx, soup):
+    """fp32 node layout: the kernel's traversal is the oracle's ordered
+    traversal step for step (same box arithmetic), so hits and per-ray
+    counts agree exactly."""
     hs, scene = soup
     assert scene.info.fast_path == 1
     rays = mixed_rays(rt, 65536, 11)
-    print("case: soup packed vs ordered, 65536 rays", flush=True)
-    got = run(gpu_ctx, scene, rays)
-    want, nv, tt, _ = O.closest_hit(oscene(hs), rays, O.TRAV_ORDERED, stats=True, threads=8)
-    assert_bit_equal(got, want)
-    assert (got["id"] >= 0).mean() > 0.1
-    # per-ray work counters equal the oracle's canonical counts
-    hits, gnv, gtt = gpu_ctx.closest_hit_stats(scene, rt.rays_to_device(rays))
-    torch.cuda.synchronize()
-    assert np.array_equal(gnv.cpu().numpy().astype(np.uint32), nv)
-    assert np.array_equal(gtt.cpu().numpy().astype(np.uint32), tt)
+    print("case: soup packed fp32 vs ordered, 65536 rays", flush=True)
+    gpu_ctx.set_tuning(node_layout=rt.LAYOUT_F32)
+    try:
+        got = run(gpu_ctx, scene, rays)
+        want, nv, tt, _ = O.closest_hit(oscene(hs), rays, O.TRAV_ORDERED, stats=True, threads=8)
+        assert_bit_equal(got, want)
+        assert (got["id"] >= 0).mean() > 0.1
+        hits, gnv, gtt = gpu_ctx.closest_hit_stats(scene, rt.rays_to_device(rays))
+        torch.cuda.synchronize()
+        assert np.array_equal(gnv.cpu().numpy().astype(np.uint32), nv)
+        assert np.array_equal(gtt.cpu().numpy().astype(np.uint32), tt)
+    finally:
+        gpu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("layout,lo,hi", [(2, 1.0, 1.3), (3, 0.3, 0.8)])
+def test_soup_quantized_nodes(rt, gpu_ctx, soup, layout, lo, hi):
+    """16-bit node boxes (binary 32-byte or 4-wide 64-byte records) contain
+    the exact ones: same closest hits (up to exact ties); the binary layout
+    reads a few more records than the exact-box traversal, the 4-wide one
+    about half as many."""
+    hs, scene = soup
+    rays = mixed_rays(rt, 65536, 11)
+    print(f"case: soup packed layout {layout} vs ordered", flush=True)
+    gpu_ctx.set_tuning(node_layout=layout)
+    try:
+        got = run(gpu_ctx, scene, rays)
+        want, nv, tt, _ = O.closest_hit(oscene(hs), rays, O.TRAV_ORDERED, stats=True, threads=8)
+        assert assert_equal_up_to_ties(oscene(hs), rays, got, want) <= 2
+        _, gnv, gtt = gpu_ctx.closest_hit_stats(scene, rt.rays_to_device(rays))
+        torch.cuda.synchronize()
+        ratio = gnv.cpu().numpy().astype(np.float64).sum() / nv.astype(np.float64).sum()
+        print(f"layout {layout}: node records / exact binary records {ratio:.4f}", flush=True)
+        assert lo <= ratio < hi
+    finally:
+        gpu_ctx.set_tuning()
 
 
 def test_soup_packed_vs_shader_order(rt, gpu_ctx, soup):
@@ -106,7 +135,7 @@ def test_spheres_closest_of_both(rt, gpu_ctx, soup):
     print("case: soup spheres", flush=True)
     want = O.closest_hit(oscene(hs), rays, O.TRAV_ORDERED, spheres=1, threads=8)
     assert (want["id"] >= len(hs.triangles)).any()
-    assert_bit_equal(run(gpu_ctx, scene, rays, spheres=1), want)
+    assert assert_equal_up_to_ties(oscene(hs), rays, run(gpu_ctx, scene, rays, spheres=1), want) <= 2
     want_ref = O.closest_hit(oscene(hs), rays, O.TRAV_REFERENCE, spheres=1, threads=8)
     assert_bit_equal(run(gpu_ctx, scene, rays, spheres=1, traversal=rtamd.TRAVERSE_REFERENCE), want_ref)
 
@@ -127,7 +156,7 @@ def test_edge_rays(rt, gpu_ctx, soup):
     r["o"][:512] = (t0["v0"] + t0["v1"] + t0["v2"]) / 3          # start on a triangle
     print("case: soup edge rays", flush=True)
     want = O.closest_hit(oscene(hs), r, O.TRAV_ORDERED, threads=8)
-    assert_bit_equal(run(gpu_ctx, scene, r), want)
+    assert assert_equal_up_to_ties(oscene(hs), r, run(gpu_ctx, scene, r), want) <= 2
     ref = O.closest_hit(oscene(hs), r, O.TRAV_REFERENCE, threads=8)
     assert_equal_up_to_ties(oscene(hs), r, run(gpu_ctx, scene, r), ref)
 
@@ -198,7 +227,7 @@ def test_icosphere_one_million_rays(rt, gpu_ctx):
     sample = np.concatenate([np.arange(0, 524288, 32), np.arange(524288, 1048576, 32)])
     print("case: icosphere sample vs oracle", flush=True)
     want = O.closest_hit(oscene(hs), rays[sample], O.TRAV_ORDERED, threads=8)
-    assert_bit_equal(got[sample], want)
+    assert assert_equal_up_to_ties(oscene(hs), rays[sample], got[sample], want) <= 5
     assert_equal_up_to_ties(oscene(hs), rays[sample], got[sample],
                             O.closest_hit(oscene(hs), rays[sample], O.TRAV_REFERENCE, threads=8))
     scene.release()

@@ -90,6 +90,20 @@ extern "C" int rt_destroy(rt_ctx* ctx) {
     return RT_OK;
 }
 
+extern "C" int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t) {
+    if (!ctx || !t) return RT_ERR_INVALID;
+    if (t->node_layout > RT_LAYOUT_W4) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: unknown node layout");
+    if (t->refill > 64) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: refill must be 0..64");
+    ctx->tuning = *t;
+    return RT_OK;
+}
+
+extern "C" int rt_tuning_get(const rt_ctx* ctx, rt_tuning* t) {
+    if (!ctx || !t) return RT_ERR_INVALID;
+    *t = ctx->tuning;
+    return RT_OK;
+}
+
 extern "C" const char* rt_last_error(const rt_ctx* ctx) {
     if (!ctx) return "null context";
     return ctx->err.c_str();

@@ -48,6 +48,32 @@ struct alignas(16) PNode { float4 a, b, c; int4 d; };
 struct alignas(16) PTri { float4 a, b, c; };
 static_assert(sizeof(PNode) == 64 && sizeof(PTri) == 48, "packed records");
 
+// PNodeQ (32 bytes, two 16-byte loads): the same two-children record with the
+// box coordinates as 16-bit fixed point over the scene's node bounds,
+// world = origin + q * scale per axis.  Lower bounds are rounded down and
+// upper bounds up, each by one extra quantum, so a quantized box always
+// contains the exact one and the test stays conservative.
+//   w[0] = lo0.x | lo0.y << 16    w[3] = lo1.x | lo1.y << 16
+//   w[1] = lo0.z | hi0.x << 16    w[4] = lo1.z | hi1.x << 16
+//   w[2] = hi0.y | hi0.z << 16    w[5] = hi1.y | hi1.z << 16
+//   ref0, ref1 as in PNode
+struct alignas(16) PNodeQ { uint32_t w[6]; int32_t ref0, ref1; };
+static_assert(sizeof(PNodeQ) == 32, "quantized node record");
+
+// PNode4 (64 bytes, four 16-byte loads): the binary tree collapsed two
+// levels at a time -- the children of a node's children, up to four, with
+// 16-bit boxes in the same fixed-point frame as PNodeQ.  Child k's box takes
+// w[3k..3k+2] in the PNodeQ packing; ref[k] as in PNode, kNoChild for an
+// unused slot.  One record (one 64-byte line) decides two tree levels.
+struct alignas(16) PNode4 { uint32_t w[12]; int32_t ref[4]; };
+static_assert(sizeof(PNode4) == 64, "4-wide node record");
+constexpr int32_t kNoChild = INT32_MIN + 1;
+
+// Fixed-point frame of the quantized nodes, with 1/scale for the rays.
+struct QFrame { float ox, oy, oz, sx, sy, sz, rx, ry, rz; };
+
+enum { kLayoutF32 = 1, kLayoutQ16 = 2, kLayoutW4 = 3 };
+
 constexpr int32_t kEmptyRoot = INT32_MIN;  // tree with no reachable triangle
 
 // Bytes each traversal step reads, used for the roofline accounting.
@@ -59,6 +85,11 @@ struct PackedTree {
     std::vector<PNode> nodes;
     std::vector<PTri> tris;
     std::vector<int32_t> tri_id;   // slot -> original triangle index
+    std::vector<PNodeQ> qnodes;    // the same nodes, 16-bit fixed point
+    std::vector<PNode4> w4nodes;   // the same tree, 4-wide
+    int32_t root4 = kEmptyRoot;
+    uint32_t height4 = 0;          // 4-wide node depth
+    QFrame frame{};
     int32_t root = kEmptyRoot;
     uint32_t height = 0;           // packed internal-node depth (stack bound)
 };
@@ -74,6 +105,8 @@ struct SceneCheck {
 // Host helpers (rt_scene.cpp).
 SceneCheck check_scene(const rt_scene_desc& d);
 void pack_tree(const rt_scene_desc& d, PackedTree& out);
+void quantize_nodes(PackedTree& out);
+void widen_nodes(PackedTree& out);
 
 }  // namespace rtamd
 
@@ -86,6 +119,11 @@ struct rt_scene {
     rt_light* d_lights = nullptr;
     rt_sphere* d_spheres = nullptr;
     rtamd::PNode* d_pnodes = nullptr;
+    rtamd::PNodeQ* d_qnodes = nullptr;
+    rtamd::PNode4* d_w4nodes = nullptr;
+    rtamd::QFrame frame{};
+    int32_t root4 = rtamd::kEmptyRoot;
+    uint32_t height4 = 0;
     rtamd::PTri* d_ptris = nullptr;
     int32_t* d_ptri_id = nullptr;
     int32_t root_ref = rtamd::kEmptyRoot;
@@ -106,7 +144,7 @@ struct rt_ctx {
     uint32_t* d_counters = nullptr;   // small zero-filled counter block
     uint32_t* d_cursors = nullptr;    // ring of persistent-kernel ray cursors
     uint32_t cursor_next = 0;
-    uint32_t tune_refill = 0;         // lanes idle before a wave refetches (0 = default)
+    rt_tuning tuning{};               // 0 fields = defaults
     uint64_t last_primary = 0, last_secondary = 0;
 };
 
@@ -137,6 +175,7 @@ struct TraceArgs {
     bool render = false;        // a bounce of the render's wavefront loop
 };
 int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
+int resolved_layout(const rt_ctx* ctx);   // RT_LAYOUT_* actually used
 int launch_trace_reference(rt_ctx* ctx, const TraceArgs& a, uint32_t max_stack, hipStream_t s);
 int launch_fill_u32(rt_ctx* ctx, uint32_t* p, uint32_t value, uint32_t count, hipStream_t s);
 

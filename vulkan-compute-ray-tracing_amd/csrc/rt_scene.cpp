@@ -191,6 +191,105 @@ void pack_tree(const rt_scene_desc& d, PackedTree& out) {
         else if (f.slot == 0) out.nodes[f.parent].d.x = ref;
         else out.nodes[f.parent].d.y = ref;
     }
+    quantize_nodes(out);
+    widen_nodes(out);
+}
+
+void widen_nodes(PackedTree& out) {
+    out.w4nodes.clear();
+    out.w4nodes.reserve(out.nodes.size() / 2 + 1);
+    out.height4 = 0;
+    if (out.root == kEmptyRoot || out.root < 0) { out.root4 = out.root; return; }
+    // the binary record i holds its two children's boxes (q16) and refs;
+    // the 4-wide record of i takes its grandchildren through internal
+    // children and the leaf children themselves
+    struct Item { uint32_t w0, w1, w2; int32_t ref; };
+    auto child = [&](int32_t bi, int k) -> Item {
+        const PNodeQ& q = out.qnodes[bi];
+        return k == 0 ? Item{q.w[0], q.w[1], q.w[2], q.ref0} : Item{q.w[3], q.w[4], q.w[5], q.ref1};
+    };
+    struct Frame { int32_t bi; int32_t parent; int slot; uint32_t depth; };
+    std::vector<Frame> st;
+    st.push_back({out.root, -1, 0, 1});
+    while (!st.empty()) {
+        Frame f = st.back();
+        st.pop_back();
+        Item items[4];
+        int n = 0;
+        for (int k = 0; k < 2; ++k) {
+            Item c = child(f.bi, k);
+            if (c.ref >= 0) {
+                items[n++] = child(c.ref, 0);
+                items[n++] = child(c.ref, 1);
+            } else {
+                items[n++] = c;
+            }
+        }
+        const int32_t me = (int32_t)out.w4nodes.size();
+        out.w4nodes.push_back(PNode4{});
+        out.height4 = std::max(out.height4, f.depth);
+        PNode4& w = out.w4nodes[me];
+        for (int k = 0; k < 4; ++k) {
+            if (k < n) {
+                w.w[3 * k] = items[k].w0;
+                w.w[3 * k + 1] = items[k].w1;
+                w.w[3 * k + 2] = items[k].w2;
+                w.ref[k] = items[k].ref;     // patched below for internal children
+            } else {
+                w.w[3 * k] = w.w[3 * k + 1] = w.w[3 * k + 2] = 0;
+                w.ref[k] = kNoChild;
+            }
+        }
+        for (int k = n - 1; k >= 0; --k)
+            if (items[k].ref >= 0) st.push_back({items[k].ref, me, k, f.depth + 1});
+        if (f.parent < 0) out.root4 = me;
+        else out.w4nodes[f.parent].ref[f.slot] = me;
+    }
+}
+
+void quantize_nodes(PackedTree& out) {
+    out.qnodes.assign(out.nodes.size(), PNodeQ{});
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (const PNode& p : out.nodes) {
+        const float c[12] = {p.a.x, p.a.y, p.a.z, p.a.w, p.b.x, p.b.y, p.b.z, p.b.w, p.c.x, p.c.y, p.c.z, p.c.w};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], (double)std::min(c[k], c[6 + k]));
+            hi[k] = std::max(hi[k], (double)std::max(c[3 + k], c[9 + k]));
+        }
+    }
+    if (out.nodes.empty()) for (int k = 0; k < 3; ++k) lo[k] = hi[k] = 0.0;
+    float org[3], scl[3];
+    for (int k = 0; k < 3; ++k) {
+        org[k] = (float)lo[k];
+        if ((double)org[k] > lo[k]) org[k] = std::nextafter(org[k], -INFINITY);
+        double ext = std::max(hi[k] - (double)org[k], 1e-30);
+        scl[k] = (float)(ext / 65000.0);
+        if ((double)scl[k] * 65000.0 < ext) scl[k] = std::nextafter(scl[k], INFINITY);
+    }
+    out.frame = QFrame{org[0], org[1], org[2], scl[0], scl[1], scl[2], 1.0f / scl[0], 1.0f / scl[1], 1.0f / scl[2]};
+    // q covers [org + q*scl] exactly (double arithmetic of the float frame)
+    auto qlo = [&](float v, int k) -> uint32_t {
+        double q = std::floor(((double)v - (double)org[k]) / (double)scl[k]) - 1.0;
+        return (uint32_t)std::min(65535.0, std::max(0.0, q));
+    };
+    auto qhi = [&](float v, int k) -> uint32_t {
+        double q = std::ceil(((double)v - (double)org[k]) / (double)scl[k]) + 1.0;
+        return (uint32_t)std::min(65535.0, std::max(0.0, q));
+    };
+    for (size_t i = 0; i < out.nodes.size(); ++i) {
+        const PNode& p = out.nodes[i];
+        const float l0[3] = {p.a.x, p.a.y, p.a.z}, h0[3] = {p.a.w, p.b.x, p.b.y};
+        const float l1[3] = {p.b.z, p.b.w, p.c.x}, h1[3] = {p.c.y, p.c.z, p.c.w};
+        PNodeQ& q = out.qnodes[i];
+        q.w[0] = qlo(l0[0], 0) | qlo(l0[1], 1) << 16;
+        q.w[1] = qlo(l0[2], 2) | qhi(h0[0], 0) << 16;
+        q.w[2] = qhi(h0[1], 1) | qhi(h0[2], 2) << 16;
+        q.w[3] = qlo(l1[0], 0) | qlo(l1[1], 1) << 16;
+        q.w[4] = qlo(l1[2], 2) | qhi(h1[0], 0) << 16;
+        q.w[5] = qhi(h1[1], 1) | qhi(h1[2], 2) << 16;
+        q.ref0 = p.d.x;
+        q.ref1 = p.d.y;
+    }
 }
 
 }  // namespace rtamd
@@ -214,7 +313,9 @@ int upload_array(rt_ctx* ctx, T** dst, const T* src, uint32_t n, uint64_t& bytes
 void free_scene(rt_scene* s) {
     if (!s) return;
     for (void* p : {(void*)s->d_tris, (void*)s->d_mats, (void*)s->d_nodes, (void*)s->d_lights,
-                    (void*)s->d_spheres, (void*)s->d_pnodes, (void*)s->d_ptris, (void*)s->d_ptri_id})
+                    (void*)s->d_spheres, (void*)s->d_pnodes, (void*)s->d_qnodes, (void*)s->d_w4nodes,
+                    (void*)s->d_ptris,
+                    (void*)s->d_ptri_id})
         if (p) (void)hipFree(p);
     delete s;
 }
@@ -256,6 +357,8 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene*
         if ((rc = upload_array(ctx, &s->d_spheres, desc->spheres, desc->num_spheres, bytes))) break;
         if (fast) {
             if ((rc = upload_array(ctx, &s->d_pnodes, packed.nodes.data(), (uint32_t)packed.nodes.size(), bytes))) break;
+            if ((rc = upload_array(ctx, &s->d_qnodes, packed.qnodes.data(), (uint32_t)packed.qnodes.size(), bytes))) break;
+            if ((rc = upload_array(ctx, &s->d_w4nodes, packed.w4nodes.data(), (uint32_t)packed.w4nodes.size(), bytes))) break;
             if ((rc = upload_array(ctx, &s->d_ptris, packed.tris.data(), (uint32_t)packed.tris.size(), bytes))) break;
             if ((rc = upload_array(ctx, &s->d_ptri_id, packed.tri_id.data(), (uint32_t)packed.tri_id.size(), bytes))) break;
         }
@@ -265,6 +368,9 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene*
         return rc;
     }
     s->root_ref = fast ? packed.root : kEmptyRoot;
+    s->frame = packed.frame;
+    s->root4 = fast ? packed.root4 : kEmptyRoot;
+    s->height4 = fast ? packed.height4 : 0;
     s->packed_height = fast ? packed.height : 0;
     s->ref_height = chk.height;
     s->fast = fast;

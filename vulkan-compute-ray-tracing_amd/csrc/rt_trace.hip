@@ -111,20 +111,56 @@ __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
+// Scene and launch arguments of the persistent traversal, by value.
+struct TraceScene {
+    const PNode* nodes;       // kLayoutF32
+    const PNodeQ* qnodes;     // kLayoutQ16
+    const PNode4* w4nodes;    // kLayoutW4
+    const PTri* tris;
+    const int32_t* tri_id;
+    const rt_sphere* spheres;
+    QFrame frame;
+    int32_t root;
+    uint32_t nspheres;
+    uint32_t ntri;
+};
+struct TraceIo {
+    const rt_ray* rays;
+    const uint32_t* n_dev;
+    rt_hit* hits;
+    uint32_t* node_visits;
+    uint32_t* tri_tests;
+    uint32_t* cursor;
+    uint32_t n_host;
+    uint32_t refill;
+};
+
+__device__ __forceinline__ float lo16(uint32_t w) { return (float)(w & 0xffffu); }
+
+// Compare-exchange of (key, value) pairs: afterwards ka <= kb.
+__device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t& vb) {
+    bool s = kb < ka;
+    float k = s ? kb : ka;
+    kb = s ? ka : kb;
+    ka = k;
+    int32_t v = s ? vb : va;
+    vb = s ? va : vb;
+    va = v;
+}
+__device__ __forceinline__ float hi16(uint32_t w) { return (float)(w >> 16); }
+
 // Persistent traversal with per-lane ray fetch.  Each lane owns one ray at a
 // time; when at least `refill` lanes of the wave are idle, the wave takes
 // that many ray indices from the global cursor with one atomic and hands
 // them out by ballot prefix count, so a long ray never holds its wave's
 // other lanes idle.  One loop iteration reads one node record or one
-// triangle record per busy lane.
-template <int BLOCK, int STACK, bool STATS>
-__device__ __forceinline__ void trace_persistent(
-    const PNode* __restrict__ nodes, const PTri* __restrict__ tris, const int32_t* __restrict__ tri_id,
-    int32_t root, const rt_ray* __restrict__ rays, uint32_t n_host, const uint32_t* __restrict__ n_dev,
-    rt_hit* __restrict__ hits, const rt_sphere* __restrict__ spheres, uint32_t nspheres, uint32_t ntri,
-    uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests, uint32_t* __restrict__ cursor,
-    uint32_t refill, int32_t* __restrict__ stk) {
-    const uint32_t n = n_dev ? *n_dev : n_host;
+// triangle record per busy lane.  With 16-bit nodes the ray is moved once
+// into the nodes' fixed-point frame (origin (o - org) / scale, reciprocal
+// direction times scale): slab distances t are the same numbers, so tmin,
+// tmax and the closest t apply unchanged.
+template <int BLOCK, int STACK, int LAYOUT, bool STATS>
+__device__ __forceinline__ void trace_persistent(const TraceScene& sc, const TraceIo& io, int32_t* __restrict__ stk) {
+    const uint32_t n = io.n_dev ? *io.n_dev : io.n_host;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
     const uint64_t lanes_below = (1ull << lane) - 1ull;
@@ -133,7 +169,8 @@ __device__ __forceinline__ void trace_persistent(
     bool drained = false;           // wave-uniform: the cursor passed n
     uint32_t ray_i = 0;
     RayIn r{};
-    f3 inv{0.0f, 0.0f, 0.0f};
+    f3 bo{0.0f, 0.0f, 0.0f};        // ray origin in the box frame
+    f3 inv{0.0f, 0.0f, 0.0f};       // reciprocal direction in the box frame
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
     int32_t best = -1;
     uint32_t nv = 0, nt = 0;
@@ -144,23 +181,29 @@ __device__ __forceinline__ void trace_persistent(
         // ---- refill idle lanes
         const uint64_t idle = __ballot(!busy);
         const uint32_t n_idle = (uint32_t)__popcll(idle);
-        if (!drained && n_idle >= refill) {
+        if (!drained && n_idle >= io.refill) {
             const uint32_t leader = (uint32_t)__builtin_ctzll(idle);
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(cursor, n_idle);
+            if (lane == leader) base = atomicAdd(io.cursor, n_idle);
             base = __shfl(base, (int)leader);
             if (base + n_idle >= n) drained = true;
             if (!busy) {
                 uint32_t mine = base + (uint32_t)__popcll(idle & lanes_below);
                 if (mine < n) {
                     ray_i = mine;
-                    r = load_ray(rays, mine);
+                    r = load_ray(io.rays, mine);
                     inv = f3{safe_inv(r.d.x), safe_inv(r.d.y), safe_inv(r.d.z)};
+                    bo = r.o;
+                    if (LAYOUT == kLayoutQ16 || LAYOUT == kLayoutW4) {
+                        const QFrame& q = sc.frame;
+                        bo = f3{(r.o.x - q.ox) * q.rx, (r.o.y - q.oy) * q.ry, (r.o.z - q.oz) * q.rz};
+                        inv = f3{inv.x * q.sx, inv.y * q.sy, inv.z * q.sz};
+                    }
                     best_t = r.tmax; bu = 0.0f; bv = 0.0f; best = -1;
                     nv = 0; nt = 0; sp = 0;
-                    cur = root;
+                    cur = sc.root;
                     busy = true;
-                    if (root == kEmptyRoot) cur = 0, sp = -1;   // finishes on the first step
+                    if (sc.root == kEmptyRoot) cur = 0, sp = -1;   // finishes on the first step
                 }
             }
         }
@@ -172,27 +215,73 @@ __device__ __forceinline__ void trace_persistent(
         if (busy) {
             bool pop = true;
             if (sp >= 0) {
-                if (cur >= 0) {
-                    const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-                    float4 a = np[0], b = np[1], c = np[2];
-                    int4 dd = reinterpret_cast<const int4*>(np)[3];
+                if (cur >= 0 && LAYOUT == kLayoutW4) {
+                    const uint4* np = reinterpret_cast<const uint4*>(sc.w4nodes + cur);
+                    uint4 a = np[0], b = np[1], c = np[2];
+                    int4 d = reinterpret_cast<const int4*>(np)[3];
                     if (STATS) nv++;
+                    float k0 = __builtin_inff(), k1 = __builtin_inff(), k2 = __builtin_inff(), k3 = __builtin_inff();
+                    float tn;
+                    int nh = 0;
+                    if (d.x != kNoChild && slab(lo16(a.x), hi16(a.x), lo16(a.y), hi16(a.y), lo16(a.z), hi16(a.z), bo, inv,
+                                                r.tmin, best_t, tn)) { k0 = tn; nh++; }
+                    if (d.y != kNoChild && slab(lo16(a.w), hi16(a.w), lo16(b.x), hi16(b.x), lo16(b.y), hi16(b.y), bo, inv,
+                                                r.tmin, best_t, tn)) { k1 = tn; nh++; }
+                    if (d.z != kNoChild && slab(lo16(b.z), hi16(b.z), lo16(b.w), hi16(b.w), lo16(c.x), hi16(c.x), bo, inv,
+                                                r.tmin, best_t, tn)) { k2 = tn; nh++; }
+                    if (d.w != kNoChild && slab(lo16(c.y), hi16(c.y), lo16(c.z), hi16(c.z), lo16(c.w), hi16(c.w), bo, inv,
+                                                r.tmin, best_t, tn)) { k3 = tn; nh++; }
+                    if (nh > 0) {
+                        int32_t v0 = d.x, v1 = d.y, v2 = d.z, v3 = d.w;
+                        // sort the four (entry distance, child) pairs; misses carry +inf
+                        cswap(k0, v0, k1, v1);
+                        cswap(k2, v2, k3, v3);
+                        cswap(k0, v0, k2, v2);
+                        cswap(k1, v1, k3, v3);
+                        cswap(k1, v1, k2, v2);
+                        // farthest pushed first so the nearest deferred child pops first
+                        if (nh > 3) { stk[sp * BLOCK + tid] = v3; sp++; }
+                        if (nh > 2) { stk[sp * BLOCK + tid] = v2; sp++; }
+                        if (nh > 1) { stk[sp * BLOCK + tid] = v1; sp++; }
+                        cur = v0;
+                        pop = false;
+                    }
+                } else if (cur >= 0) {
+                    bool h0, h1;
                     float tn0, tn1;
-                    bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, r.o, inv, r.tmin, best_t, tn0);
-                    bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, r.o, inv, r.tmin, best_t, tn1);
+                    int32_t c0, c1;
+                    if (LAYOUT == kLayoutQ16) {
+                        const uint4* np = reinterpret_cast<const uint4*>(sc.qnodes + cur);
+                        uint4 a = np[0], b = np[1];
+                        h0 = slab(lo16(a.x), hi16(a.x), lo16(a.y), hi16(a.y), lo16(a.z), hi16(a.z), bo, inv,
+                                  r.tmin, best_t, tn0);
+                        h1 = slab(lo16(a.w), hi16(a.w), lo16(b.x), hi16(b.x), lo16(b.y), hi16(b.y), bo, inv,
+                                  r.tmin, best_t, tn1);
+                        c0 = (int32_t)b.z;
+                        c1 = (int32_t)b.w;
+                    } else {
+                        const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
+                        float4 a = np[0], b = np[1], c = np[2];
+                        int4 dd = reinterpret_cast<const int4*>(np)[3];
+                        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, bo, inv, r.tmin, best_t, tn0);
+                        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, bo, inv, r.tmin, best_t, tn1);
+                        c0 = dd.x;
+                        c1 = dd.y;
+                    }
+                    if (STATS) nv++;
                     if (h0 && h1) {
                         bool swap = tn1 < tn0;
-                        stk[sp * BLOCK + tid] = swap ? dd.x : dd.y;
+                        stk[sp * BLOCK + tid] = swap ? c0 : c1;
                         sp++;
-                        cur = swap ? dd.y : dd.x;
+                        cur = swap ? c1 : c0;
                         pop = false;
                     } else if (h0 || h1) {
-                        cur = h0 ? dd.x : dd.y;
+                        cur = h0 ? c0 : c1;
                         pop = false;
                     }
                 } else {
                     const int32_t slot = ~cur;
-                    const float4* tp = reinterpret_cast<const float4*>(tris + slot);
+                    const float4* tp = reinterpret_cast<const float4*>(sc.tris + slot);
                     PTri tr{tp[0], tp[1], tp[2]};
                     if (STATS) nt++;
                     float t, u, v;
@@ -207,20 +296,20 @@ __device__ __forceinline__ void trace_persistent(
                     cur = stk[sp * BLOCK + tid];
                 } else {
                     // ---- ray finished: spheres, then the result
-                    int32_t id = best >= 0 ? tri_id[best] : -1;
-                    if (nspheres) {
+                    int32_t id = best >= 0 ? sc.tri_id[best] : -1;
+                    if (sc.nspheres) {
                         float closest = best >= 0 ? best_t : r.tmax;
-                        for (uint32_t j = 0; j < nspheres; ++j) {
+                        for (uint32_t j = 0; j < sc.nspheres; ++j) {
                             float root_t; int back;
-                            if (sphere_test(spheres[j], r.o, r.d, r.tmin, closest, root_t, back)) {
-                                closest = root_t; best_t = root_t; id = (int32_t)(ntri + j);
+                            if (sphere_test(sc.spheres[j], r.o, r.d, r.tmin, closest, root_t, back)) {
+                                closest = root_t; best_t = root_t; id = (int32_t)(sc.ntri + j);
                                 bu = (float)back; bv = 0.0f;
                             }
                         }
                     }
-                    if (id >= 0) store_hit(hits, ray_i, id, best_t, bu, bv);
-                    else store_hit(hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
-                    if (STATS) { node_visits[ray_i] = nv; tri_tests[ray_i] = nt; }
+                    if (id >= 0) store_hit(io.hits, ray_i, id, best_t, bu, bv);
+                    else store_hit(io.hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
+                    if (STATS) { io.node_visits[ray_i] = nv; io.tri_tests[ray_i] = nt; }
                     busy = false;
                 }
             }
@@ -228,27 +317,18 @@ __device__ __forceinline__ void trace_persistent(
     }
 }
 
-#define RT_TRACE_PARAMS                                                                                        \
-    const PNode *__restrict__ nodes, const PTri *__restrict__ tris, const int32_t *__restrict__ tri_id,        \
-        int32_t root, const rt_ray *__restrict__ rays, uint32_t n_host, const uint32_t *__restrict__ n_dev,     \
-        rt_hit *__restrict__ hits, const rt_sphere *__restrict__ spheres, uint32_t nspheres, uint32_t ntri,     \
-        uint32_t *__restrict__ node_visits, uint32_t *__restrict__ tri_tests, uint32_t *__restrict__ cursor,    \
-        uint32_t refill
-#define RT_TRACE_ARGS \
-    nodes, tris, tri_id, root, rays, n_host, n_dev, hits, spheres, nspheres, ntri, node_visits, tri_tests, cursor, refill
-
 // Closest-hit queries (rt_closest_hit, rt_closest_hit_stats).
-template <int BLOCK, int STACK, bool STATS>
-__global__ __launch_bounds__(BLOCK) void k_trace_packed(RT_TRACE_PARAMS) {
+template <int BLOCK, int STACK, int LAYOUT, bool STATS>
+__global__ __launch_bounds__(BLOCK) void k_trace_packed(TraceScene sc, TraceIo io) {
     __shared__ int32_t stk[STACK * BLOCK];
-    trace_persistent<BLOCK, STACK, STATS>(RT_TRACE_ARGS, stk);
+    trace_persistent<BLOCK, STACK, LAYOUT, STATS>(sc, io, stk);
 }
 
 // The render's per-bounce traces over the live-ray queue (count on device).
-template <int BLOCK, int STACK>
-__global__ __launch_bounds__(BLOCK) void k_trace_render(RT_TRACE_PARAMS) {
+template <int BLOCK, int STACK, int LAYOUT>
+__global__ __launch_bounds__(BLOCK) void k_trace_render(TraceScene sc, TraceIo io) {
     __shared__ int32_t stk[STACK * BLOCK];
-    trace_persistent<BLOCK, STACK, false>(RT_TRACE_ARGS, stk);
+    trace_persistent<BLOCK, STACK, LAYOUT, false>(sc, io, stk);
 }
 
 // intersectAABB (ray-trace-compute.comp:250-258) with GLSL min/max.
@@ -343,15 +423,16 @@ uint32_t* next_cursor(rt_ctx* ctx) {
     return ctx->d_cursors + k;
 }
 
-template <int STACK>
+template <int STACK, int LAYOUT>
 int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
     const rt_scene* sc = a.scene;
-    auto kern = a.render ? k_trace_render<kBlock, STACK>
-                : stats    ? k_trace_packed<kBlock, STACK, true>
-                           : k_trace_packed<kBlock, STACK, false>;
+    auto kern = a.render ? k_trace_render<kBlock, STACK, LAYOUT>
+                : stats    ? k_trace_packed<kBlock, STACK, LAYOUT, true>
+                           : k_trace_packed<kBlock, STACK, LAYOUT, false>;
     int per_cu = 0;
     RT_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
     if (per_cu < 1) per_cu = 1;
+    if (ctx->tuning.blocks_per_cu && (int)ctx->tuning.blocks_per_cu < per_cu) per_cu = (int)ctx->tuning.blocks_per_cu;
     uint32_t blocks = (uint32_t)ctx->num_cus * (uint32_t)per_cu;
     if (!a.n_dev) {
         // no more waves than rays
@@ -363,28 +444,53 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
     int rc = launch_fill_u32(ctx, cursor, 0u, 1u, s);
     if (rc) return rc;
     // 1..64 idle lanes trigger a refetch (64: only when the whole wave is idle)
-    uint32_t refill = ctx->tune_refill ? ctx->tune_refill : 16u;
+    uint32_t refill = ctx->tuning.refill ? ctx->tuning.refill : 32u;
     if (refill > 64u) refill = 64u;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, sc->d_pnodes, sc->d_ptris, sc->d_ptri_id,
-                       sc->root_ref, a.rays, a.n, a.n_dev, a.hits, sc->d_spheres,
-                       a.spheres ? sc->info.num_spheres : 0u, sc->info.num_triangles, a.node_visits, a.tri_tests,
-                       cursor, refill);
+    TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->d_spheres, sc->frame,
+                  LAYOUT == kLayoutW4 ? sc->root4 : sc->root_ref, a.spheres ? sc->info.num_spheres : 0u,
+                  sc->info.num_triangles};
+    TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill};
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, ts, io);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;
 }
 
+template <int STACK>
+int launch_packed_layout(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
+    if (ctx->tuning.node_layout == RT_LAYOUT_F32) return launch_packed_t<STACK, kLayoutF32>(ctx, a, s, stats);
+    return launch_packed_t<STACK, kLayoutQ16>(ctx, a, s, stats);
+}
+
 }  // namespace
+
+int resolved_layout(const rt_ctx* ctx) {
+    uint32_t l = ctx->tuning.node_layout;
+    return l == RT_LAYOUT_AUTO ? (int)RT_LAYOUT_W4 : (int)l;
+}
 
 int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s) {
     const rt_scene* sc = a.scene;
     if (!sc->fast) return set_error(ctx, RT_ERR_UNSUPPORTED, "packed traversal needs a tree that encloses its triangles");
     const bool stats = a.node_visits && a.tri_tests;
+    if (resolved_layout(ctx) == RT_LAYOUT_W4) {
+        // up to three deferred children per 4-wide node on the path
+        const uint32_t need = 3u * sc->height4 + 1u;
+        if (need <= 24) return launch_packed_t<24, kLayoutW4>(ctx, a, s, stats);
+        if (need <= 36) return launch_packed_t<36, kLayoutW4>(ctx, a, s, stats);
+        if (need <= 48) return launch_packed_t<48, kLayoutW4>(ctx, a, s, stats);
+        if (need <= 64) return launch_packed_t<64, kLayoutW4>(ctx, a, s, stats);
+        if (need <= 96) return launch_packed_t<96, kLayoutW4>(ctx, a, s, stats);
+        return set_error(ctx, RT_ERR_UNSUPPORTED, "4-wide tree deeper than 31 levels");
+    }
     const uint32_t h = sc->packed_height;
-    if (h <= 16) return launch_packed_t<16>(ctx, a, s, stats);
-    if (h <= 24) return launch_packed_t<24>(ctx, a, s, stats);
-    if (h <= 32) return launch_packed_t<32>(ctx, a, s, stats);
-    if (h <= 48) return launch_packed_t<48>(ctx, a, s, stats);
-    if (h <= 64) return launch_packed_t<64>(ctx, a, s, stats);
+    if (h <= 16) return launch_packed_layout<16>(ctx, a, s, stats);
+    if (h <= 20) return launch_packed_layout<20>(ctx, a, s, stats);
+    if (h <= 22) return launch_packed_layout<22>(ctx, a, s, stats);
+    if (h <= 24) return launch_packed_layout<24>(ctx, a, s, stats);
+    if (h <= 28) return launch_packed_layout<28>(ctx, a, s, stats);
+    if (h <= 32) return launch_packed_layout<32>(ctx, a, s, stats);
+    if (h <= 48) return launch_packed_layout<48>(ctx, a, s, stats);
+    if (h <= 64) return launch_packed_layout<64>(ctx, a, s, stats);
     return set_error(ctx, RT_ERR_UNSUPPORTED, "packed tree deeper than 64 levels");
 }
 

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "librtamd.so")
 RT_OK, RT_ERR_INVALID, RT_ERR_HIP, RT_ERR_NOMEM, RT_ERR_UNSUPPORTED, RT_ERR_BUSY, RT_ERR_CANCELLED, RT_ERR_NOT_FOUND = range(8)
 RT_MAT_LIGHT, RT_MAT_LAMBERTIAN, RT_MAT_METAL, RT_MAT_GLASS = range(4)
 TRAVERSE_AUTO, TRAVERSE_PACKED, TRAVERSE_REFERENCE = range(3)
+LAYOUT_AUTO, LAYOUT_F32, LAYOUT_Q16, LAYOUT_W4 = range(4)
 
 # --------------------------------------------------------------- records
 TRI_DT = np.dtype({"names": ["v0", "v1", "v2", "materialIndex"],
@@ -67,6 +68,11 @@ class RenderParams(ctypes.Structure):
                 ("traversal", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("node_layout", ctypes.c_uint32), ("refill", ctypes.c_uint32),
+                ("blocks_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
+
+
 class Ubo(ctypes.Structure):
     _fields_ = [("camPos", ctypes.c_float * 3), ("time", ctypes.c_float),
                 ("currentSample", ctypes.c_uint32), ("numTriangles", ctypes.c_uint32),
@@ -92,7 +98,7 @@ _lib = None
 # Every symbol include/rtamd.h and include/rtamd_workloads.h declare.
 EXPORTS = [
     "rt_create", "rt_destroy", "rt_last_error", "rt_status_string", "rt_scene_upload", "rt_scene_info_get",
-    "rt_scene_check",
+    "rt_scene_check", "rt_tuning_set", "rt_tuning_get",
     "rt_scene_release", "rt_closest_hit", "rt_closest_hit_stats", "rt_render", "rt_render_frames",
     "rt_render_last_rays", "rt_bvh_build", "rt_bvh_build_seeded", "rt_scene_lights",
     "rtw_scene_soup", "rtw_scene_icosphere", "rtw_scene_clusters", "rtw_scene_free", "rtw_rays_camera_tiles",
@@ -118,6 +124,8 @@ def lib():
         "rt_scene_info_get": (i32, [P, ctypes.POINTER(SceneInfo)]),
         "rt_scene_release": (i32, [P, P]),
         "rt_scene_check": (i32, [ctypes.POINTER(SceneDesc), ctypes.POINTER(SceneInfo), ctypes.c_char_p, ctypes.c_size_t]),
+        "rt_tuning_set": (i32, [P, ctypes.POINTER(Tuning)]),
+        "rt_tuning_get": (i32, [P, ctypes.POINTER(Tuning)]),
         "rt_closest_hit": (i32, [P, P, P, u32, P, ctypes.POINTER(QueryParams), P]),
         "rt_closest_hit_stats": (i32, [P, P, P, u32, P, P, P, P]),
         "rt_render": (i32, [P, P, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), P, P, u32, u32, P]),
@@ -309,6 +317,15 @@ class Context:
 
     def check(self, rc, what=""):
         _check(rc, self._h, what)
+
+    def set_tuning(self, node_layout=0, refill=0, blocks_per_cu=0):
+        t = Tuning(node_layout, refill, blocks_per_cu)
+        self.check(lib().rt_tuning_set(self._h, ctypes.byref(t)), "rt_tuning_set")
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        self.check(lib().rt_tuning_get(self._h, ctypes.byref(t)), "rt_tuning_get")
+        return t
 
     def upload(self, scene: HostScene) -> "Scene":
         h = ctypes.c_void_p()
