@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--layout", choices=["w4", "q16", "f32"], default="w4",
                     help="node records: 4-wide 64-byte 16-bit (default), binary 32-byte 16-bit, binary 64-byte fp32")
     ap.add_argument("--refill", type=int, default=0, help="idle lanes before a wave refetches (0 = library default)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the measurement, copy a buffer of this many MiB once (a known-size kernel for "
+                         "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard by batch with no "
@@ -153,6 +156,15 @@ def main():
         out["render"] = {"workload": "1920x1080, 4 spp (currentSample 0..3), 8 bounces, vfov 60, eye (0,0,10)",
                          "ms": round(rms, 3), "rays": prim + sec, "secondary_rays": sec,
                          "mrays_per_s": round((prim + sec) / (rms / 1e3) / 1e6, 2)}
+
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = src.numel() * 4
+        del src, dst
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hs, rays_np, args.cpu_seconds, threads)
