@@ -176,10 +176,58 @@ typedef struct rt_tuning {
                                nodes, or 4-wide 64-byte 16-bit nodes            */
     uint32_t refill;        /* idle lanes (1..64) that make a wave fetch rays     */
     uint32_t blocks_per_cu; /* persistent blocks per CU (0 = occupancy limit)     */
-    uint32_t reserved[5];
+    uint32_t stack_lds;     /* 4-wide layout: 0 = whole traversal stack in LDS;
+                               8 or 16 = that many top entries in LDS, deeper ones
+                               spill to HBM (more waves fit per CU)               */
+    uint32_t reserved[4];
 } rt_tuning;
 int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t);
 int rt_tuning_get(const rt_ctx* ctx, rt_tuning* t);
+
+/* ----------------------------------------------------------------- batched */
+/* Many (scene, ray batch) items traced by ONE persistent launch with a
+ * global work queue, optionally regrouping each item's rays by direction
+ * octant and origin cell first, replayed from a hipGraph.  Each item gives
+ * the same hits as rt_closest_hit on it alone (triangles only; the scenes
+ * must take the packed path).  No counterpart in the reference, which
+ * dispatches one image of one scene per frame (main.cpp:228). */
+typedef struct rt_batch_item {
+    const rt_scene* scene;
+    const rt_ray* rays_dev;
+    rt_hit* hits_dev;
+    uint32_t count;
+    uint32_t reserved;
+} rt_batch_item;
+
+typedef struct rt_batch_config {
+    uint32_t block;          /* threads per work-group: 128 or 256 (0 = 256)       */
+    uint32_t stack_lds;      /* 0 = whole stack in LDS; 8 or 16 = top entries only  */
+    uint32_t regroup;        /* 1 = bin each item's rays by octant and origin cell  */
+    uint32_t refill;         /* idle lanes (1..64) before a wave fetches rays       */
+    uint32_t blocks_per_cu;  /* 0 = occupancy limit                                 */
+    uint32_t use_graph;      /* 1 = replay the launch sequence from a hipGraph      */
+    uint32_t reserved[2];
+} rt_batch_config;
+
+typedef struct rt_batch rt_batch;
+
+/* Builds the device item table and work queue for `n` items (their ray and
+ * hit buffers are the caller's) and captures the launch sequence.  NULL
+ * config = defaults (256, 0, 0, 32, 0, 1). */
+int rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, uint32_t n, const rt_batch_config* config,
+                    rt_batch** out);
+int rt_batch_set_config(rt_ctx* ctx, rt_batch* batch, const rt_batch_config* config);
+int rt_batch_get_config(const rt_batch* batch, rt_batch_config* config);
+/* Enqueues the whole batch on `stream`. */
+int rt_batch_launch(rt_ctx* ctx, rt_batch* batch, void* stream);
+/* Times candidate configurations on this batch (coordinate descent over
+ * block, stack_lds, regroup, refill and blocks_per_cu; `trials` timed launches
+ * each), applies the fastest and returns it.  With a cache file, a stored
+ * result for the same shape (item count, log2 rays, log2 triangles) is reused
+ * and new results are appended. */
+int rt_batch_autotune(rt_ctx* ctx, rt_batch* batch, uint32_t trials, const char* cache_path,
+                      rt_batch_config* best);
+int rt_batch_destroy(rt_ctx* ctx, rt_batch* batch);
 
 /* ------------------------------------------------------------------ render */
 typedef struct rt_render_params {

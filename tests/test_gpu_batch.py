@@ -1,0 +1,113 @@
+"""GPU tests of the batched form (rt_batch_*): one persistent launch over
+many (scene, ray batch) items must give every item exactly the hits the
+plain kernel gives it alone, for every launch configuration."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def mixed(rt, gpu_ctx):
+    """Six scenes of 1k..120k triangles, seven items (one scene twice, one
+    empty batch, counts not multiples of 64)."""
+    sizes = [1000, 3000, 17000, 60000, 120000, 5000]
+    scenes = [gpu_ctx.upload(rt.scene_clusters(n, 100 + i)) for i, n in enumerate(sizes)]
+    counts = [4096, 777, 0, 20000, 65536, 3001, 12345]
+    which = [0, 1, 2, 3, 4, 5, 3]
+    items, refs = [], []
+    for k, (c, w) in enumerate(zip(counts, which)):
+        rays = rt.rays_to_device(rt.rays_box(c, 200 + k, 4.0)) if c else torch.zeros((0, 8), device="cuda")
+        hits = torch.full((c, 4), -7.0, device="cuda")
+        items.append((scenes[w], rays, hits))
+        refs.append(rt.hits_from_device(gpu_ctx.closest_hit(scenes[w], rays)) if c else None)
+    torch.cuda.synchronize()
+    return scenes, items, refs
+
+
+def check(rt, items, refs):
+    torch.cuda.synchronize()
+    for (scene, rays, hits), ref in zip(items, refs):
+        if ref is None:
+            continue
+        got = rt.hits_from_device(hits)
+        assert np.array_equal(got["id"], ref["id"])
+        for f in ("t", "u", "v"):
+            assert np.array_equal(bits(got[f]), bits(ref[f]))
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(),
+    dict(use_graph=0),
+    dict(regroup=1),
+    dict(stack_lds=8),
+    dict(stack_lds=16, regroup=1),
+    dict(block=128, refill=16),
+    dict(blocks_per_cu=2, refill=64),
+])
+def test_batch_matches_plain(rt, gpu_ctx, mixed, cfg):
+    _, items, refs = mixed
+    print(f"case: batch config {cfg}", flush=True)
+    for _, _, hits in items:
+        hits.fill_(-7.0)
+    b = rt.Batch(gpu_ctx, items, rt.batch_config(**cfg))
+    b.launch()
+    check(rt, items, refs)
+    b.launch()                      # replay
+    check(rt, items, refs)
+    b.destroy()
+
+
+def test_batch_autotune_and_cache(rt, gpu_ctx, mixed, tmp_path):
+    _, items, refs = mixed
+    print("case: batch autotune", flush=True)
+    b = rt.Batch(gpu_ctx, items)
+    cache = str(tmp_path / "tune.txt")
+    best = b.autotune(trials=2, cache_path=cache)
+    assert best.block in (128, 256) and best.stack_lds in (0, 8, 16) and 1 <= best.refill <= 64
+    assert os.path.getsize(cache) > 0
+    b.destroy()
+    b2 = rt.Batch(gpu_ctx, items)
+    again = b2.autotune(trials=2, cache_path=cache)           # read back from the cache
+    assert again.as_dict() == {**best.as_dict(), "use_graph": again.use_graph}
+    for _, _, hits in items:
+        hits.fill_(-7.0)
+    b2.launch()
+    check(rt, items, refs)
+    b2.destroy()
+
+
+def test_plain_ring_stack_matches_full_stack(rt, gpu_ctx):
+    hs = rt.scene_clusters(200000, 9)
+    scene = gpu_ctx.upload(hs)
+    rays = rt.rays_to_device(rt.rays_box(50000, 3, 4.0))
+    print("case: plain ring stacks", flush=True)
+    ref = rt.hits_from_device(gpu_ctx.closest_hit(scene, rays))
+    for ring in (8, 16):
+        gpu_ctx.set_tuning(stack_lds=ring)
+        try:
+            got = rt.hits_from_device(gpu_ctx.closest_hit(scene, rays))
+        finally:
+            gpu_ctx.set_tuning()
+        assert np.array_equal(got["id"], ref["id"]) and np.array_equal(bits(got["t"]), bits(ref["t"]))
+    scene.release()
+
+
+def test_batch_rejects_scene_without_packed_path(rt, gpu_ctx):
+    hs = rt.scene_soup(64, 2)
+    leaf = np.nonzero(hs.nodes["object"] >= 0)[0][:4]
+    hs.nodes["max"][leaf] = hs.nodes["min"][leaf]
+    scene = gpu_ctx.upload(hs)
+    rays = rt.rays_to_device(rt.rays_box(128, 1, 1.0))
+    hits = torch.empty((128, 4), device="cuda")
+    with pytest.raises(rt.RtError) as e:
+        rt.Batch(gpu_ctx, [(scene, rays, hits)])
+    assert e.value.status == rt.RT_ERR_UNSUPPORTED
+    scene.release()

@@ -29,6 +29,7 @@ SOURCES = [
     "rt_workloads.cpp",
     "rt_trace.hip",
     "rt_render.hip",
+    "rt_batch.hip",
 ]
 
 COMMON = [

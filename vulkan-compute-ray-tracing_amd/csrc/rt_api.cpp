@@ -82,7 +82,7 @@ extern "C" int rt_destroy(rt_ctx* ctx) {
     if (!ctx) return RT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 4; ++i)
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_cursors) (void)hipFree(ctx->d_cursors);
@@ -94,6 +94,8 @@ extern "C" int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t) {
     if (!ctx || !t) return RT_ERR_INVALID;
     if (t->node_layout > RT_LAYOUT_W4) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: unknown node layout");
     if (t->refill > 64) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: refill must be 0..64");
+    if (t->stack_lds != 0 && t->stack_lds != 8 && t->stack_lds != 16)
+        return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: stack_lds must be 0, 8 or 16");
     ctx->tuning = *t;
     return RT_OK;
 }

@@ -138,9 +138,9 @@ struct rt_ctx {
     std::string err;
     int num_cus = 0;
     // grow-only device workspaces: [0] render path state, [1] reference
-    // traversal stacks, [2] statistics scratch
-    void* ws[3] = {nullptr, nullptr, nullptr};
-    size_t ws_bytes[3] = {0, 0, 0};
+    // traversal stacks, [2] statistics scratch, [3] traversal stack spill
+    void* ws[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t ws_bytes[4] = {0, 0, 0, 0};
     uint32_t* d_counters = nullptr;   // small zero-filled counter block
     uint32_t* d_cursors = nullptr;    // ring of persistent-kernel ray cursors
     uint32_t cursor_next = 0;
@@ -152,7 +152,7 @@ namespace rtamd {
 
 int set_error(rt_ctx* ctx, int code, const std::string& msg);
 int hip_error(rt_ctx* ctx, hipError_t e, const char* what);
-enum { kWsRender = 0, kWsStack = 1, kWsStats = 2 };
+enum { kWsRender = 0, kWsStack = 1, kWsStats = 2, kWsSpill = 3 };
 constexpr uint32_t kCursorSlots = 1024;
 int ensure_workspace(rt_ctx* ctx, int slot, size_t bytes);
 

@@ -1,0 +1,374 @@
+// rt_trace_dev.hpp -- device-side traversal shared by the plain query,
+// render and batched kernels: ray/hit records, the conservative slab test,
+// the exact triangle and sphere tests of the shader, and the persistent
+// ordered traversal loop.
+#pragma once
+
+#include "rt_common.hpp"
+#include "rt_math.hpp"
+
+#pragma clang fp contract(off)
+
+namespace rtamd {
+
+struct RayIn {
+    f3 o, d;
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ RayIn load_ray(const rt_ray* rays, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(rays) + 2 * (size_t)i;
+    float4 a = p[0], b = p[1];
+    return RayIn{f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, a.w, b.w};
+}
+
+__device__ __forceinline__ void store_hit(rt_hit* hits, uint32_t i, int32_t id, float t, float u, float v) {
+    float4 h = make_float4(__int_as_float(id), t, u, v);
+    reinterpret_cast<float4*>(hits)[i] = h;
+}
+
+// Conservative slab test of one box against [tmin, tmax].  Rounding of the
+// reciprocal form is absorbed by a relative slack of 2^-16 (far larger than
+// the few-ulp error), so a box that holds an accepted hit is never culled.
+__device__ __forceinline__ bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz,
+                                     const f3& o, const f3& inv, float tmin, float tmax, float& tnear) {
+    float tx0 = (lox - o.x) * inv.x, tx1 = (hix - o.x) * inv.x;
+    float ty0 = (loy - o.y) * inv.y, ty1 = (hiy - o.y) * inv.y;
+    float tz0 = (loz - o.z) * inv.z, tz1 = (hiz - o.z) * inv.z;
+    float nearv = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    float farv = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    tnear = nearv;
+    return nearv <= farv + (fabsf(farv) + fabsf(nearv)) * 0x1p-16f;
+}
+
+// triIntersect + hit_triangle's accept test on a packed triangle.
+__device__ __forceinline__ bool tri_test(const PTri& tr, const f3& o, const f3& d, float tmin, float tbest,
+                                         float& t, float& u, float& v) {
+    f3 v0{tr.a.x, tr.a.y, tr.a.z};
+    f3 a{tr.a.w, tr.b.x, tr.b.y};
+    f3 b{tr.b.z, tr.b.w, tr.c.x};
+    f3 n{tr.c.y, tr.c.z, tr.c.w};
+    f3 p = v0 - o;
+    f3 q = cross(p, d);
+    float idet = 1.0f / dot(d, n);
+    u = dot(q, b) * idet;
+    v = dot(q, a) * idet;
+    t = dot(n, p) * idet;
+    bool inside = !(u < 0.0f || u > 1.0f || v < 0.0f || (u + v) > 1.0f);
+    return inside && t > tmin && t < tbest;
+}
+
+// hit_sphere (ray-trace-compute.comp:175-203): returns the accepted root
+// and whether it is the far one (backFaceInt).
+__device__ __forceinline__ bool sphere_test(const rt_sphere& s, const f3& o, const f3& d, float tmin, float tmax,
+                                            float& root, int& back) {
+    f3 center{s.s[0], s.s[1], s.s[2]};
+    float radius = s.s[3];
+    f3 oc = o - center;
+    float a = dot(d, d);
+    float half_b = dot(oc, d);
+    float c = dot(oc, oc) - radius * radius;
+    float disc = half_b * half_b - a * c;
+    if (disc < 0.0f) return false;
+    float sqrtd = sqrtf(disc);
+    back = 0;
+    root = (-half_b - sqrtd) / a;
+    if (root < tmin || tmax < root) {
+        root = (-half_b + sqrtd) / a;
+        back = 1;
+        if (root < tmin || tmax < root) return false;
+    }
+    return true;
+}
+
+// Reciprocal for the box tests: a zero (or tiny) direction component
+// becomes +-2^-80 so every slab bound stays finite (an infinite bound would
+// make the slack term infinite and keep every box).  The ray this describes
+// departs from the true one by < 1e-20 over t <= 1e4, far inside the boxes'
+// padding, so the test stays conservative.
+__device__ __forceinline__ float safe_inv(float d) {
+    const float tiny = 0x1p-80f;
+    float a = fabsf(d) < tiny ? copysignf(tiny, d) : d;
+    return 1.0f / a;
+}
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Scene and launch arguments of the persistent traversal, by value.
+struct TraceScene {
+    const PNode* nodes;       // kLayoutF32
+    const PNodeQ* qnodes;     // kLayoutQ16
+    const PNode4* w4nodes;    // kLayoutW4
+    const PTri* tris;
+    const int32_t* tri_id;
+    const rt_sphere* spheres;
+    QFrame frame;
+    int32_t root;
+    uint32_t nspheres;
+    uint32_t ntri;
+};
+struct TraceIo {
+    const rt_ray* rays;
+    const uint32_t* n_dev;
+    rt_hit* hits;
+    uint32_t* node_visits;
+    uint32_t* tri_tests;
+    uint32_t* cursor;
+    uint32_t n_host;
+    uint32_t refill;
+};
+
+__device__ __forceinline__ float lo16(uint32_t w) { return (float)(w & 0xffffu); }
+
+// Compare-exchange of (key, value) pairs: afterwards ka <= kb.
+__device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t& vb) {
+    bool s = kb < ka;
+    float k = s ? kb : ka;
+    kb = s ? ka : kb;
+    ka = k;
+    int32_t v = s ? vb : va;
+    vb = s ? va : vb;
+    va = v;
+}
+__device__ __forceinline__ float hi16(uint32_t w) { return (float)(w >> 16); }
+
+// One item of a batched launch: a resident scene's 4-wide tree and a ray
+// batch with its hit buffer.  Global ray g of the launch belongs to the item
+// with base <= g < base + count.
+struct BatchEntry {
+    const PNode4* nodes;
+    const PTri* tris;
+    const int32_t* tri_id;
+    const rt_ray* rays;
+    rt_hit* hits;
+    QFrame frame;
+    int32_t root;
+    uint32_t base;
+    uint32_t count;
+    uint32_t pad;
+};
+struct BatchIo {
+    const BatchEntry* table;
+    const uint32_t* offsets;      // n_items + 1 prefix sums of counts
+    const uint32_t* chunk_item;   // item of global ray 64*c
+    const uint32_t* perm;         // optional: global ray -> ray index inside its item
+};
+
+// Persistent traversal with per-lane ray fetch.  Each lane owns one ray at a
+// time; when at least `refill` lanes of the wave are idle, the wave takes
+// that many ray indices from the global cursor with one atomic and hands
+// them out by ballot prefix count, so a long ray never holds its wave's
+// other lanes idle.  One loop iteration reads one node record or one
+// triangle record per busy lane.  With 16-bit nodes the ray is moved once
+// into the nodes' fixed-point frame (origin (o - org) / scale, reciprocal
+// direction times scale): slab distances t are the same numbers, so tmin,
+// tmax and the closest t apply unchanged.
+//
+// BATCHED: rays come from many items (BatchIo); each lane carries its own
+// scene pointers.  RING > 0: only the top RING stack entries of a lane live
+// in LDS (a ring); deeper entries move to `spill` in HBM, one word per
+// push/pop beyond RING.  RING = 0 keeps the whole stack (STACK entries) in LDS.
+template <int BLOCK, int STACK, int LAYOUT, bool STATS, bool BATCHED, int RING>
+__device__ __forceinline__ void trace_persistent(const TraceScene& sc, const TraceIo& io, const BatchIo& bio,
+                                                 int32_t* __restrict__ stk, int32_t* __restrict__ spill) {
+    static_assert(!BATCHED || LAYOUT == kLayoutW4, "batched launches use the 4-wide layout");
+    static_assert(RING == 0 || (RING & (RING - 1)) == 0, "ring size is a power of two");
+    const uint32_t n = io.n_dev ? *io.n_dev : io.n_host;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = lane_id();
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+    const size_t gstride = (size_t)gridDim.x * BLOCK;
+    const size_t gtid = (size_t)blockIdx.x * BLOCK + tid;
+
+    bool busy = false;
+    bool drained = false;           // wave-uniform: the cursor passed n
+    uint32_t ray_i = 0;
+    RayIn r{};
+    f3 bo{0.0f, 0.0f, 0.0f};        // ray origin in the box frame
+    f3 inv{0.0f, 0.0f, 0.0f};       // reciprocal direction in the box frame
+    float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
+    int32_t best = -1;
+    uint32_t nv = 0, nt = 0;
+    int sp = 0;
+    int32_t cur = 0;
+    // the lane's scene (uniform unless BATCHED)
+    const PNode4* l_w4 = sc.w4nodes;
+    const PTri* l_tris = sc.tris;
+    const int32_t* l_triid = sc.tri_id;
+    rt_hit* l_hits = io.hits;
+
+    auto push = [&](int32_t v) {
+        if (RING == 0) {
+            stk[sp * BLOCK + tid] = v;
+        } else {
+            const int slot = sp & (RING - 1);
+            if (sp >= RING) spill[(size_t)(sp - RING) * gstride + gtid] = stk[slot * BLOCK + tid];
+            stk[slot * BLOCK + tid] = v;
+        }
+        sp++;
+    };
+    auto pop = [&]() -> int32_t {
+        sp--;
+        if (RING == 0) return stk[sp * BLOCK + tid];
+        const int slot = sp & (RING - 1);
+        int32_t v = stk[slot * BLOCK + tid];
+        if (sp >= RING) stk[slot * BLOCK + tid] = spill[(size_t)(sp - RING) * gstride + gtid];
+        return v;
+    };
+
+    for (;;) {
+        // ---- refill idle lanes
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!drained && n_idle >= io.refill) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(idle);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(io.cursor, n_idle);
+            base = __shfl(base, (int)leader);
+            if (base + n_idle >= n) drained = true;
+            if (!busy) {
+                uint32_t mine = base + (uint32_t)__popcll(idle & lanes_below);
+                if (mine < n) {
+                    QFrame q = sc.frame;
+                    int32_t root = sc.root;
+                    if (BATCHED) {
+                        uint32_t b = bio.chunk_item[mine >> 6];
+                        while (mine >= bio.offsets[b + 1]) ++b;
+                        const BatchEntry* e = bio.table + b;
+                        l_w4 = e->nodes;
+                        l_tris = e->tris;
+                        l_triid = e->tri_id;
+                        l_hits = e->hits;
+                        root = e->root;
+                        q = e->frame;
+                        ray_i = bio.perm ? bio.perm[mine] : mine - e->base;
+                        r = load_ray(e->rays, ray_i);
+                    } else {
+                        ray_i = mine;
+                        r = load_ray(io.rays, mine);
+                    }
+                    inv = f3{safe_inv(r.d.x), safe_inv(r.d.y), safe_inv(r.d.z)};
+                    bo = r.o;
+                    if (LAYOUT == kLayoutQ16 || LAYOUT == kLayoutW4) {
+                        bo = f3{(r.o.x - q.ox) * q.rx, (r.o.y - q.oy) * q.ry, (r.o.z - q.oz) * q.rz};
+                        inv = f3{inv.x * q.sx, inv.y * q.sy, inv.z * q.sz};
+                    }
+                    best_t = r.tmax; bu = 0.0f; bv = 0.0f; best = -1;
+                    nv = 0; nt = 0; sp = 0;
+                    cur = root;
+                    busy = true;
+                    if (root == kEmptyRoot) cur = 0, sp = -1;   // finishes on the first step
+                }
+            }
+        }
+        if (__ballot(busy) == 0) {
+            if (drained) break;
+            continue;
+        }
+        // ---- one traversal step per busy lane
+        if (busy) {
+            bool next = false;          // cur holds the next record to read
+            if (sp >= 0) {
+                if (cur >= 0 && LAYOUT == kLayoutW4) {
+                    const uint4* np = reinterpret_cast<const uint4*>(l_w4 + cur);
+                    uint4 a = np[0], b = np[1], c = np[2];
+                    int4 d = reinterpret_cast<const int4*>(np)[3];
+                    if (STATS) nv++;
+                    float k0 = __builtin_inff(), k1 = __builtin_inff(), k2 = __builtin_inff(), k3 = __builtin_inff();
+                    float tn;
+                    int nh = 0;
+                    if (d.x != kNoChild && slab(lo16(a.x), hi16(a.x), lo16(a.y), hi16(a.y), lo16(a.z), hi16(a.z), bo, inv,
+                                                r.tmin, best_t, tn)) { k0 = tn; nh++; }
+                    if (d.y != kNoChild && slab(lo16(a.w), hi16(a.w), lo16(b.x), hi16(b.x), lo16(b.y), hi16(b.y), bo, inv,
+                                                r.tmin, best_t, tn)) { k1 = tn; nh++; }
+                    if (d.z != kNoChild && slab(lo16(b.z), hi16(b.z), lo16(b.w), hi16(b.w), lo16(c.x), hi16(c.x), bo, inv,
+                                                r.tmin, best_t, tn)) { k2 = tn; nh++; }
+                    if (d.w != kNoChild && slab(lo16(c.y), hi16(c.y), lo16(c.z), hi16(c.z), lo16(c.w), hi16(c.w), bo, inv,
+                                                r.tmin, best_t, tn)) { k3 = tn; nh++; }
+                    if (nh > 0) {
+                        int32_t v0 = d.x, v1 = d.y, v2 = d.z, v3 = d.w;
+                        // sort the four (entry distance, child) pairs; misses carry +inf
+                        cswap(k0, v0, k1, v1);
+                        cswap(k2, v2, k3, v3);
+                        cswap(k0, v0, k2, v2);
+                        cswap(k1, v1, k3, v3);
+                        cswap(k1, v1, k2, v2);
+                        // farthest pushed first so the nearest deferred child pops first
+                        if (nh > 3) push(v3);
+                        if (nh > 2) push(v2);
+                        if (nh > 1) push(v1);
+                        cur = v0;
+                        next = true;
+                    }
+                } else if (cur >= 0) {
+                    bool h0, h1;
+                    float tn0, tn1;
+                    int32_t c0, c1;
+                    if (LAYOUT == kLayoutQ16) {
+                        const uint4* np = reinterpret_cast<const uint4*>(sc.qnodes + cur);
+                        uint4 a = np[0], b = np[1];
+                        h0 = slab(lo16(a.x), hi16(a.x), lo16(a.y), hi16(a.y), lo16(a.z), hi16(a.z), bo, inv,
+                                  r.tmin, best_t, tn0);
+                        h1 = slab(lo16(a.w), hi16(a.w), lo16(b.x), hi16(b.x), lo16(b.y), hi16(b.y), bo, inv,
+                                  r.tmin, best_t, tn1);
+                        c0 = (int32_t)b.z;
+                        c1 = (int32_t)b.w;
+                    } else {
+                        const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
+                        float4 a = np[0], b = np[1], c = np[2];
+                        int4 dd = reinterpret_cast<const int4*>(np)[3];
+                        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, bo, inv, r.tmin, best_t, tn0);
+                        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, bo, inv, r.tmin, best_t, tn1);
+                        c0 = dd.x;
+                        c1 = dd.y;
+                    }
+                    if (STATS) nv++;
+                    if (h0 && h1) {
+                        bool swap = tn1 < tn0;
+                        push(swap ? c0 : c1);
+                        cur = swap ? c1 : c0;
+                        next = true;
+                    } else if (h0 || h1) {
+                        cur = h0 ? c0 : c1;
+                        next = true;
+                    }
+                } else {
+                    const int32_t slot = ~cur;
+                    const float4* tp = reinterpret_cast<const float4*>(l_tris + slot);
+                    PTri tr{tp[0], tp[1], tp[2]};
+                    if (STATS) nt++;
+                    float t, u, v;
+                    if (tri_test(tr, r.o, r.d, r.tmin, best_t, t, u, v)) {
+                        best_t = t; bu = u; bv = v; best = slot;
+                    }
+                }
+            }
+            if (!next) {
+                if (sp > 0) {
+                    cur = pop();
+                } else {
+                    // ---- ray finished: spheres, then the result
+                    int32_t id = best >= 0 ? l_triid[best] : -1;
+                    if (!BATCHED && sc.nspheres) {
+                        float closest = best >= 0 ? best_t : r.tmax;
+                        for (uint32_t j = 0; j < sc.nspheres; ++j) {
+                            float root_t; int back;
+                            if (sphere_test(sc.spheres[j], r.o, r.d, r.tmin, closest, root_t, back)) {
+                                closest = root_t; best_t = root_t; id = (int32_t)(sc.ntri + j);
+                                bu = (float)back; bv = 0.0f;
+                            }
+                        }
+                    }
+                    if (id >= 0) store_hit(l_hits, ray_i, id, best_t, bu, bv);
+                    else store_hit(l_hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
+                    if (STATS) { io.node_visits[ray_i] = nv; io.tri_tests[ray_i] = nt; }
+                    busy = false;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace rtamd

@@ -70,13 +70,32 @@ class RenderParams(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("node_layout", ctypes.c_uint32), ("refill", ctypes.c_uint32),
-                ("blocks_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
+                ("blocks_per_cu", ctypes.c_uint32), ("stack_lds", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
 
 
 class Ubo(ctypes.Structure):
     _fields_ = [("camPos", ctypes.c_float * 3), ("time", ctypes.c_float),
                 ("currentSample", ctypes.c_uint32), ("numTriangles", ctypes.c_uint32),
                 ("numLights", ctypes.c_uint32), ("numSpheres", ctypes.c_uint32)]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("rays_dev", ctypes.c_void_p), ("hits_dev", ctypes.c_void_p),
+                ("count", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_uint32), ("stack_lds", ctypes.c_uint32), ("regroup", ctypes.c_uint32),
+                ("refill", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+def batch_config(block=256, stack_lds=0, regroup=0, refill=32, blocks_per_cu=0, use_graph=1) -> BatchConfig:
+    return BatchConfig(block, stack_lds, regroup, refill, blocks_per_cu, use_graph)
 
 
 class WScene(ctypes.Structure):
@@ -101,6 +120,8 @@ EXPORTS = [
     "rt_scene_check", "rt_tuning_set", "rt_tuning_get",
     "rt_scene_release", "rt_closest_hit", "rt_closest_hit_stats", "rt_render", "rt_render_frames",
     "rt_render_last_rays", "rt_bvh_build", "rt_bvh_build_seeded", "rt_scene_lights",
+    "rt_batch_create", "rt_batch_set_config", "rt_batch_get_config", "rt_batch_launch", "rt_batch_autotune",
+    "rt_batch_destroy",
     "rtw_scene_soup", "rtw_scene_icosphere", "rtw_scene_clusters", "rtw_scene_free", "rtw_rays_camera_tiles",
     "rtw_rays_shell", "rtw_rays_box", "rtw_log_uniform",
 ]
@@ -131,6 +152,12 @@ def lib():
         "rt_render": (i32, [P, P, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), P, P, u32, u32, P]),
         "rt_render_frames": (i32, [P, P, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), u32, P, P, u32, u32, P]),
         "rt_render_last_rays": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+        "rt_batch_create": (i32, [P, ctypes.POINTER(BatchItem), u32, ctypes.POINTER(BatchConfig), ctypes.POINTER(P)]),
+        "rt_batch_set_config": (i32, [P, P, ctypes.POINTER(BatchConfig)]),
+        "rt_batch_get_config": (i32, [P, ctypes.POINTER(BatchConfig)]),
+        "rt_batch_launch": (i32, [P, P, P]),
+        "rt_batch_autotune": (i32, [P, P, u32, ctypes.c_char_p, ctypes.POINTER(BatchConfig)]),
+        "rt_batch_destroy": (i32, [P, P]),
         "rt_bvh_build": (i32, [P, u32, P]),
         "rt_bvh_build_seeded": (i32, [P, u32, u32, u32, P]),
         "rt_scene_lights": (i32, [P, u32, P, u32, P, u32, ctypes.POINTER(u32)]),
@@ -318,8 +345,8 @@ class Context:
     def check(self, rc, what=""):
         _check(rc, self._h, what)
 
-    def set_tuning(self, node_layout=0, refill=0, blocks_per_cu=0):
-        t = Tuning(node_layout, refill, blocks_per_cu)
+    def set_tuning(self, node_layout=0, refill=0, blocks_per_cu=0, stack_lds=0):
+        t = Tuning(node_layout, refill, blocks_per_cu, stack_lds)
         self.check(lib().rt_tuning_set(self._h, ctypes.byref(t)), "rt_tuning_set")
 
     def get_tuning(self) -> Tuning:
@@ -388,6 +415,47 @@ class Context:
         p, s = ctypes.c_uint64(), ctypes.c_uint64()
         self.check(lib().rt_render_last_rays(self._h, ctypes.byref(p), ctypes.byref(s)), "rt_render_last_rays")
         return p.value, s.value
+
+
+class Batch:
+    """An rt_batch: items (scene, rays, hits) traced by one persistent launch."""
+
+    def __init__(self, ctx: "Context", items, config: BatchConfig | None = None):
+        self.ctx = ctx
+        self._keep = list(items)
+        arr = (BatchItem * max(1, len(items)))()
+        for k, (scene, rays, hits) in enumerate(items):
+            arr[k] = BatchItem(scene.handle.value if hasattr(scene.handle, "value") else scene.handle,
+                               _dev_ptr(rays) if rays.numel() else None, _dev_ptr(hits) if hits.numel() else None,
+                               rays.shape[0], 0)
+        h = ctypes.c_void_p()
+        ctx.check(lib().rt_batch_create(ctx.handle, arr, len(items), ctypes.byref(config) if config else None,
+                                        ctypes.byref(h)), "rt_batch_create")
+        self.handle = h
+
+    def launch(self, stream=None):
+        self.ctx.check(lib().rt_batch_launch(self.ctx.handle, self.handle, _stream_ptr(stream)), "rt_batch_launch")
+
+    def set_config(self, config: BatchConfig):
+        self.ctx.check(lib().rt_batch_set_config(self.ctx.handle, self.handle, ctypes.byref(config)),
+                       "rt_batch_set_config")
+
+    def config(self) -> BatchConfig:
+        c = BatchConfig()
+        _check(lib().rt_batch_get_config(self.handle, ctypes.byref(c)), what="rt_batch_get_config")
+        return c
+
+    def autotune(self, trials=3, cache_path: str | None = None) -> BatchConfig:
+        c = BatchConfig()
+        self.ctx.check(lib().rt_batch_autotune(self.ctx.handle, self.handle, trials,
+                                               cache_path.encode() if cache_path else None, ctypes.byref(c)),
+                       "rt_batch_autotune")
+        return c
+
+    def destroy(self):
+        if self.handle:
+            self.ctx.check(lib().rt_batch_destroy(self.ctx.handle, self.handle), "rt_batch_destroy")
+            self.handle = None
 
 
 class Scene:
