@@ -263,6 +263,59 @@ int rt_render_frames(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo,
  * (one count per hit_bvh call, i.e. per ray). */
 int rt_render_last_rays(const rt_ctx* ctx, uint64_t* primary, uint64_t* secondary);
 
+/* ------------------------------------------------------------------ served */
+/* A request loop on its own thread: scenes stay resident under an HBM
+ * budget (least recently used scenes without pending work are evicted to
+ * admit a new one), closest-hit and render requests queue on arrival and
+ * are launched as device slots free up.  In RT_SERVE_BATCHED mode each slot
+ * launch traces every queued query (up to max_batch_rays) in one batched
+ * launch, and up to `slots` launches overlap on the device; RT_SERVE_FIFO
+ * runs one request at a time with the plain kernel.  Request buffers are
+ * host memory owned by the caller until the request completes.  Latency is
+ * measured from rt_server_submit_* to the result being in the caller's
+ * buffer.  No counterpart in the reference. */
+enum { RT_SERVE_BATCHED = 0, RT_SERVE_FIFO = 1 };
+typedef struct rt_server_config {
+    uint32_t mode;            /* RT_SERVE_*                                     */
+    uint32_t slots;           /* concurrent device launches (1..8, 0 = 3)       */
+    uint32_t max_batch_rays;  /* rays per batched launch (0 = 1,048,576)        */
+    uint32_t max_render_pixels; /* largest render request (0 = 1,048,576)       */
+    uint64_t hbm_budget;      /* bytes for resident scenes (0 = no limit)       */
+    uint32_t reserved[4];
+} rt_server_config;
+
+typedef struct rt_server_stats {
+    uint64_t submitted, completed, failed, launches;
+    uint64_t scenes_added, scenes_evicted, scenes_refused;
+    uint64_t rays;                 /* traversals done (query rays + render rays) */
+    uint64_t resident_bytes;
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* over completed requests        */
+} rt_server_stats;
+
+typedef struct rt_server rt_server;
+int rt_server_create(rt_ctx* ctx, const rt_server_config* config, rt_server** out);
+int rt_server_destroy(rt_server* server);
+/* Uploads a scene and makes it resident, evicting least recently used idle
+ * scenes while the budget is exceeded; RT_ERR_NOMEM when it cannot fit. */
+int rt_server_scene_add(rt_server* server, const rt_scene_desc* desc, uint64_t* scene_id);
+int rt_server_scene_remove(rt_server* server, uint64_t scene_id);
+/* Writes up to `cap` resident scene ids; *count receives the total. */
+int rt_server_scene_list(rt_server* server, uint64_t* ids, uint32_t cap, uint32_t* count);
+int rt_server_submit_query(rt_server* server, uint64_t scene_id, const rt_ray* rays, uint32_t n, rt_hit* hits,
+                           uint64_t* request_id);
+/* One frame (rt_render) into `target` (w*h*4 bytes, RGBA8) from a zero
+ * accumulation image. */
+int rt_server_submit_render(rt_server* server, uint64_t scene_id, const rt_ubo* ubo,
+                            const rt_render_params* params, uint32_t w, uint32_t h, uint8_t* target,
+                            uint64_t* request_id);
+/* Waits for a request (timeout_ms < 0: forever); *status gets its result. */
+int rt_server_wait(rt_server* server, uint64_t request_id, int timeout_ms, int* status);
+/* Waits until every submitted request has completed. */
+int rt_server_drain(rt_server* server);
+int rt_server_stats_get(rt_server* server, rt_server_stats* stats);
+int rt_server_stats_reset(rt_server* server);
+const char* rt_server_last_error(const rt_server* server);
+
 /* --------------------------------------------------------------- host BVH */
 /* Bvh::createBvh (src/ray-tracing/Bvh.h:141-209): 2n-1 nodes into out[].
  * Consumes the C library's rand() exactly as the reference does (one call

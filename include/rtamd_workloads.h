@@ -61,6 +61,33 @@ int rtw_rays_shell(uint32_t n, uint64_t seed, float radius, float half, rt_ray* 
 /* Uniform rays: origin U[-extent,extent]^3, direction uniform on the sphere. */
 int rtw_rays_box(uint32_t n, uint64_t seed, float extent, rt_ray* out);
 
+/* Poisson request stream against a server (the served workload): arrivals
+ * at `rate_per_s` for `duration_s`; each request is a closest-hit query of
+ * log-uniform size in [query_min, query_max] rays (origins U[-4,4]^3,
+ * uniform directions) or, with probability render_fraction, a render_w x
+ * render_h one-sample render with render_bounces bounces, of a uniformly
+ * chosen resident scene.  Every swap_interval_s a new clustered-soup scene
+ * of swap_triangles is added from a helper thread (the server evicts the
+ * least recently used scene when its budget is exceeded).  Waits for every
+ * request, then returns the server's statistics. */
+typedef struct rtw_serve_params {
+    double rate_per_s, duration_s, render_fraction, swap_interval_s;
+    uint32_t query_min, query_max;
+    uint32_t render_w, render_h, render_bounces;
+    uint32_t swap_triangles;
+    uint64_t seed;
+    uint32_t threads;            /* host threads for scene generation */
+    uint32_t reserved;
+} rtw_serve_params;
+
+typedef struct rtw_serve_result {
+    rt_server_stats stats;
+    double wall_s;
+    uint64_t requests, queries, renders, swaps;
+} rtw_serve_result;
+
+int rtw_serve_poisson(rt_server* server, const rtw_serve_params* params, rtw_serve_result* result);
+
 /* Log-uniform integer in [lo, hi]: floor(exp(U * ln(hi/lo)) * lo), U from
  * the seed's first draw (used for scene and batch sizes). */
 uint32_t rtw_log_uniform(uint64_t seed, uint32_t lo, uint32_t hi);

@@ -1,0 +1,98 @@
+"""GPU tests of the served form (rt_server_*): every request's result is the
+same alone and in the mix, scenes are evicted least-recently-used under the
+HBM budget, and a short Poisson run completes every request."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+CAM = [0.0, 10.0, 0.0]
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def scenes(rt):
+    return [rt.scene_clusters(n, 300 + i) for i, n in enumerate([20000, 70000, 150000])]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_results_same_alone_and_in_mix(rt, gpu_ctx, scenes, mode):
+    srv = rt.Server(gpu_ctx, mode=mode, slots=3, max_batch_rays=1 << 18, max_render_pixels=128 * 128)
+    ids = [srv.add_scene(hs) for hs in scenes]
+    alone = [gpu_ctx.upload(hs) for hs in scenes]
+    rng = np.random.default_rng(4)
+    reqs = []
+    for k in range(48):
+        s = int(rng.integers(0, 3))
+        if k % 6 == 5:
+            tgt = np.zeros((96, 128, 4), np.uint8)
+            reqs.append(("render", s, tgt, srv.submit_render(ids[s], CAM, 128, 96, tgt, bounces=4)))
+        else:
+            n = int(rng.integers(1, 40000))
+            rays = rt.rays_box(n, 500 + k, 4.0)
+            hits = np.zeros(n, rt.HIT_DT)
+            reqs.append(("query", s, (rays, hits), srv.submit_query(ids[s], rays, hits)))
+    print(f"case: server mode {mode}, 48 requests", flush=True)
+    for kind, s, data, rid in reqs:
+        assert srv.wait(rid, 60000) == rt.RT_OK
+    for kind, s, data, rid in reqs:
+        if kind == "query":
+            rays, hits = data
+            ref = rt.hits_from_device(gpu_ctx.closest_hit(alone[s], rt.rays_to_device(rays)))
+            assert np.array_equal(hits["id"], ref["id"])
+            for f in ("t", "u", "v"):
+                assert np.array_equal(bits(hits[f]), bits(ref[f]))
+        else:
+            ref = gpu_ctx.render(alone[s], CAM, 128, 96, vfov=60.0, bounces=4).cpu().numpy()
+            assert np.array_equal(data, ref)
+    st = srv.stats()
+    assert st.completed == 48 and st.failed == 0 and st.p99_ms >= st.p50_ms > 0
+    if mode == 0:
+        assert st.launches < 48            # queries were batched
+    srv.close()
+    for a in alone:
+        a.release()
+
+
+def test_lru_eviction_under_budget(rt, gpu_ctx, scenes):
+    probe = gpu_ctx.upload(scenes[0])
+    b0 = probe.info.device_bytes
+    probe.release()
+    srv = rt.Server(gpu_ctx, slots=2, hbm_budget=int(b0 * 2.5))
+    a = srv.add_scene(scenes[0])
+    b = srv.add_scene(scenes[0])
+    rays = rt.rays_box(1000, 1, 4.0)
+    hits = np.zeros(1000, rt.HIT_DT)
+    srv.wait(srv.submit_query(a, rays, hits))          # a used after b: b is least recent
+    c = srv.add_scene(scenes[0])
+    assert sorted(srv.scenes()) == sorted([a, c])
+    st = srv.stats()
+    assert st.scenes_evicted == 1 and st.resident_bytes <= int(b0 * 2.5)
+    with pytest.raises(rt.RtError) as e:
+        srv.submit_query(b, rays, hits)
+    assert e.value.status == rt.RT_ERR_NOT_FOUND
+    srv.close()
+    tiny = rt.Server(gpu_ctx, slots=1, hbm_budget=16)    # no scene fits
+    with pytest.raises(rt.RtError) as e:
+        tiny.add_scene(scenes[0])
+    assert e.value.status == rt.RT_ERR_NOMEM
+    tiny.close()
+
+
+def test_short_poisson_run(rt, gpu_ctx, scenes):
+    srv = rt.Server(gpu_ctx, slots=3)
+    for hs in scenes:
+        srv.add_scene(hs)
+    print("case: poisson 500/s for 2 s", flush=True)
+    r = srv.poisson(rate=500.0, duration=2.0, render_fraction=0.1, swap_interval=1.0, query_max=65536,
+                    render_w=128, render_h=128, swap_triangles=50000, threads=8)
+    st = r.stats
+    print(f"requests {r.requests} completed {st.completed} p50 {st.p50_ms:.2f} p99 {st.p99_ms:.2f} ms "
+          f"launches {st.launches} swaps {r.swaps}", flush=True)
+    assert r.requests > 500 and st.completed == r.requests and st.failed == 0
+    assert r.swaps >= 1 and st.rays > 0
+    srv.close()

@@ -30,6 +30,7 @@ SOURCES = [
     "rt_trace.hip",
     "rt_render.hip",
     "rt_batch.hip",
+    "rt_server.cpp",
 ]
 
 COMMON = [

@@ -118,7 +118,7 @@ bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 int query_common(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays, uint32_t n, rt_hit* hits,
                  const rt_query_params* params, uint32_t* nv, uint32_t* tt, void* stream) {
     if (!ctx) return RT_ERR_INVALID;
-    if (!scene || scene->ctx != ctx) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: scene not from this context");
+    if (!scene_usable(ctx, scene)) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: scene not on this context's device");
     if (n == 0) return RT_OK;
     if (!rays || !hits) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: null ray or hit buffer");
     if (!aligned16(rays) || !aligned16(hits)) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: buffers must be 16-byte aligned");
@@ -159,7 +159,7 @@ int render_common(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo, const r
                   uint32_t frames, uint8_t* target, const uint8_t* accum_in, uint8_t* accum_out, uint32_t w,
                   uint32_t h, void* stream) {
     if (!ctx) return RT_ERR_INVALID;
-    if (!scene || scene->ctx != ctx) return set_error(ctx, RT_ERR_INVALID, "render: scene not from this context");
+    if (!scene_usable(ctx, scene)) return set_error(ctx, RT_ERR_INVALID, "render: scene not on this context's device");
     if (!ubo || !params) return set_error(ctx, RT_ERR_INVALID, "render: null ubo or params");
     if (w == 0 || h == 0 || frames == 0) return RT_OK;
     if (!target || !accum_in) return set_error(ctx, RT_ERR_INVALID, "render: null image");

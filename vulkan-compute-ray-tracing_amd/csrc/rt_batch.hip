@@ -270,7 +270,37 @@ std::string cache_key(const rt_batch* b) {
     return k.str();
 }
 
+template <int STACK>
+int launch_dyn_t(rt_ctx* ctx, const BatchIo& bio, uint32_t* cursor, uint32_t total, uint32_t refill, hipStream_t s) {
+    auto kern = k_trace_batch<256, STACK, 0>;
+    int per_cu = 0;
+    RT_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+    if (per_cu < 1) per_cu = 1;
+    uint32_t blocks = (uint32_t)ctx->num_cus * (uint32_t)per_cu;
+    uint32_t need = (total + 255u) / 256u;
+    if (need < blocks) blocks = need;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, cursor, 0u, 1u);
+    TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, cursor, total, refill};
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, io, bio, (int32_t*)nullptr);
+    RT_HIP(ctx, hipGetLastError());
+    return RT_OK;
+}
+
 }  // namespace
+
+namespace rtamd {
+int launch_batch_dynamic(rt_ctx* ctx, const BatchIo& bio, uint32_t* cursor, uint32_t total, uint32_t stack_need,
+                         uint32_t refill, hipStream_t s) {
+    if (!total) return RT_OK;
+    if (refill < 1u || refill > 64u) refill = 32u;
+    if (stack_need <= 24) return launch_dyn_t<24>(ctx, bio, cursor, total, refill, s);
+    if (stack_need <= 36) return launch_dyn_t<36>(ctx, bio, cursor, total, refill, s);
+    if (stack_need <= 48) return launch_dyn_t<48>(ctx, bio, cursor, total, refill, s);
+    if (stack_need <= 64) return launch_dyn_t<64>(ctx, bio, cursor, total, refill, s);
+    if (stack_need <= 96) return launch_dyn_t<96>(ctx, bio, cursor, total, refill, s);
+    return set_error(ctx, RT_ERR_UNSUPPORTED, "batch: a 4-wide tree deeper than 31 levels");
+}
+}  // namespace rtamd
 
 extern "C" int rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, uint32_t n, const rt_batch_config* config,
                                rt_batch** out) {
@@ -284,8 +314,8 @@ extern "C" int rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, uint32_t
     uint64_t tris = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const rt_batch_item& it = items[i];
-        if (!it.scene || it.scene->ctx != ctx)
-            return set_error(ctx, RT_ERR_INVALID, "rt_batch_create: item " + std::to_string(i) + " has no scene of this context");
+        if (!scene_usable(ctx, it.scene))
+            return set_error(ctx, RT_ERR_INVALID, "rt_batch_create: item " + std::to_string(i) + " has no scene on this device");
         if (!it.scene->fast)
             return set_error(ctx, RT_ERR_UNSUPPORTED, "rt_batch_create: item " + std::to_string(i) +
                                                           " has a scene without the packed traversal");

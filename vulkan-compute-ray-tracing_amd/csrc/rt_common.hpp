@@ -74,6 +74,29 @@ struct QFrame { float ox, oy, oz, sx, sy, sz, rx, ry, rz; };
 
 enum { kLayoutF32 = 1, kLayoutQ16 = 2, kLayoutW4 = 3 };
 
+// One item of a batched launch: a resident scene's 4-wide tree and a ray
+// batch with its hit buffer.  Global ray g of the launch belongs to the item
+// with base <= g < base + count.
+struct BatchEntry {
+    const PNode4* nodes;
+    const PTri* tris;
+    const int32_t* tri_id;
+    const rt_ray* rays;
+    rt_hit* hits;
+    QFrame frame;
+    int32_t root;
+    uint32_t base;
+    uint32_t count;
+    uint32_t pad;
+};
+struct BatchIo {
+    const BatchEntry* table;
+    const uint32_t* offsets;      // n_items + 1 prefix sums of counts
+    const uint32_t* chunk_item;   // item of global ray 64*c
+    const uint32_t* perm;         // optional: global ray -> ray index inside its item
+};
+
+
 constexpr int32_t kEmptyRoot = INT32_MIN;  // tree with no reachable triangle
 
 // Bytes each traversal step reads, used for the roofline accounting.
@@ -179,4 +202,16 @@ int resolved_layout(const rt_ctx* ctx);   // RT_LAYOUT_* actually used
 int launch_trace_reference(rt_ctx* ctx, const TraceArgs& a, uint32_t max_stack, hipStream_t s);
 int launch_fill_u32(rt_ctx* ctx, uint32_t* p, uint32_t value, uint32_t count, hipStream_t s);
 
+// Batched traversal from caller-filled device tables (rt_batch.hip): the
+// server's per-slot launches.  stack_need = 3 * max 4-wide height + 1.
+int launch_batch_dynamic(rt_ctx* ctx, const BatchIo& bio, uint32_t* cursor, uint32_t total, uint32_t stack_need,
+                         uint32_t refill, hipStream_t s);
+
+// A scene is a device resource: any context on its device may trace it.
+inline bool scene_usable(const rt_ctx* c, const rt_scene* s);
+
 }  // namespace rtamd
+
+inline bool rtamd::scene_usable(const rt_ctx* c, const rt_scene* s) {
+    return c && s && s->ctx && s->ctx->device == c->device;
+}

@@ -134,28 +134,6 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t
 }
 __device__ __forceinline__ float hi16(uint32_t w) { return (float)(w >> 16); }
 
-// One item of a batched launch: a resident scene's 4-wide tree and a ray
-// batch with its hit buffer.  Global ray g of the launch belongs to the item
-// with base <= g < base + count.
-struct BatchEntry {
-    const PNode4* nodes;
-    const PTri* tris;
-    const int32_t* tri_id;
-    const rt_ray* rays;
-    rt_hit* hits;
-    QFrame frame;
-    int32_t root;
-    uint32_t base;
-    uint32_t count;
-    uint32_t pad;
-};
-struct BatchIo {
-    const BatchEntry* table;
-    const uint32_t* offsets;      // n_items + 1 prefix sums of counts
-    const uint32_t* chunk_item;   // item of global ray 64*c
-    const uint32_t* perm;         // optional: global ray -> ray index inside its item
-};
-
 // Persistent traversal with per-lane ray fetch.  Each lane owns one ray at a
 // time; when at least `refill` lanes of the wave are idle, the wave takes
 // that many ray indices from the global cursor with one atomic and hands

@@ -1,7 +1,10 @@
 // rt_workloads.cpp -- synthetic scenes and ray batches (rtamd_workloads.h).
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -316,4 +319,104 @@ extern "C" int rtw_rays_box(uint32_t n, uint64_t seed, float extent, rt_ray* out
         q.d[0] = d.x; q.d[1] = d.y; q.d[2] = d.z; q.tmax = 10000.0f;
     }
     return RT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Poisson request stream against a server (rtamd_workloads.h).
+extern "C" int rtw_serve_poisson(rt_server* S, const rtw_serve_params* p, rtw_serve_result* res) {
+    using Clock = std::chrono::steady_clock;
+    if (!S || !p || !res || p->rate_per_s <= 0.0 || p->duration_s <= 0.0 || p->query_min == 0 ||
+        p->query_max < p->query_min || p->render_w == 0 || p->render_h == 0)
+        return RT_ERR_INVALID;
+    std::memset(res, 0, sizeof *res);
+    rt_server_stats_reset(S);
+    SplitMix rng(p->seed);
+    std::vector<rt_ray> pool(p->query_max);
+    rtw_rays_box(p->query_max, p->seed ^ 0x5eedull, 4.0f, pool.data());
+    const int kRing = 32;
+    std::vector<std::vector<rt_hit>> hit_ring(kRing, std::vector<rt_hit>(p->query_max));
+    std::vector<std::vector<uint8_t>> tgt_ring(kRing, std::vector<uint8_t>((size_t)p->render_w * p->render_h * 4));
+
+    std::mutex ids_mu;
+    std::vector<uint64_t> ids;
+    auto refresh = [&] {
+        uint32_t cnt = 0;
+        rt_server_scene_list(S, nullptr, 0, &cnt);
+        std::vector<uint64_t> v(cnt);
+        rt_server_scene_list(S, v.data(), cnt, &cnt);
+        v.resize(std::min<size_t>(v.size(), cnt));
+        std::lock_guard<std::mutex> lk(ids_mu);
+        ids.swap(v);
+    };
+    refresh();
+    if (ids.empty()) return RT_ERR_NOT_FOUND;
+
+    const auto t_start = Clock::now();
+    std::atomic<bool> done{false};
+    std::atomic<uint64_t> swaps{0};
+    std::thread swapper([&] {
+        if (p->swap_interval_s <= 0.0 || p->swap_triangles == 0) return;
+        uint32_t k = 0;
+        auto next = t_start + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(p->swap_interval_s));
+        while (!done.load()) {
+            while (!done.load() && Clock::now() < next) std::this_thread::sleep_for(std::chrono::milliseconds(2));
+            if (done.load()) break;
+            rtw_scene ws;
+            if (rtw_scene_clusters(p->swap_triangles, p->seed + 1000 + k, std::max(1u, p->threads), &ws) == RT_OK) {
+                rt_scene_desc d{ws.triangles, ws.num_triangles, ws.materials, ws.num_materials, ws.nodes, ws.num_nodes,
+                                ws.lights, ws.num_lights, ws.spheres, ws.num_spheres};
+                uint64_t id = 0;
+                if (rt_server_scene_add(S, &d, &id) == RT_OK) swaps++;
+                rtw_scene_free(&ws);
+                refresh();
+            }
+            ++k;
+            next += std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(p->swap_interval_s));
+        }
+    });
+
+    double t = 0.0;
+    uint64_t k = 0;
+    const double lq = std::log((double)p->query_max / (double)p->query_min);
+    for (;;) {
+        double u = ((double)(rng.next() >> 11) + 1.0) * 0x1p-53;
+        t += -std::log(u) / p->rate_per_s;
+        if (t >= p->duration_s) break;
+        std::this_thread::sleep_until(t_start + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(t)));
+        const bool render = (double)(rng.next() >> 11) * 0x1p-53 < p->render_fraction;
+        const double us = (double)(rng.next() >> 11) * 0x1p-53;
+        const double un = (double)(rng.next() >> 11) * 0x1p-53;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            uint64_t id;
+            {
+                std::lock_guard<std::mutex> lk(ids_mu);
+                if (ids.empty()) break;
+                id = ids[std::min(ids.size() - 1, (size_t)(us * (double)ids.size()))];
+            }
+            int rc;
+            if (render) {
+                rt_ubo ubo{};
+                ubo.camPos[1] = 10.0f;            // eye at (0, 0, 10)
+                rt_render_params rp{};
+                rp.vfov_deg = 60.0f;
+                rp.bounces = p->render_bounces;
+                rc = rt_server_submit_render(S, id, &ubo, &rp, p->render_w, p->render_h,
+                                             tgt_ring[k % kRing].data(), nullptr);
+            } else {
+                uint32_t n = (uint32_t)std::floor(std::exp(un * lq) * (double)p->query_min);
+                n = std::max(p->query_min, std::min(p->query_max, n));
+                rc = rt_server_submit_query(S, id, pool.data(), n, hit_ring[k % kRing].data(), nullptr);
+            }
+            if (rc == RT_ERR_NOT_FOUND) { refresh(); continue; }   // the scene was just evicted
+            if (rc == RT_OK) { res->requests++; (render ? res->renders : res->queries)++; }
+            break;
+        }
+        ++k;
+    }
+    done = true;
+    swapper.join();
+    rt_server_drain(S);
+    res->wall_s = std::chrono::duration<double>(Clock::now() - t_start).count();
+    res->swaps = swaps.load();
+    return rt_server_stats_get(S, &res->stats);
 }

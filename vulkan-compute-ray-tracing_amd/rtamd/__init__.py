@@ -98,6 +98,37 @@ def batch_config(block=256, stack_lds=0, regroup=0, refill=32, blocks_per_cu=0, 
     return BatchConfig(block, stack_lds, regroup, refill, blocks_per_cu, use_graph)
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_uint32), ("slots", ctypes.c_uint32), ("max_batch_rays", ctypes.c_uint32),
+                ("max_render_pixels", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint32 * 4)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "failed", "launches", "scenes_added",
+                                               "scenes_evicted", "scenes_refused", "rays", "resident_bytes")] + \
+               [(k, ctypes.c_double) for k in ("p50_ms", "p99_ms", "mean_ms", "max_ms")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeParams(ctypes.Structure):
+    _fields_ = [("rate_per_s", ctypes.c_double), ("duration_s", ctypes.c_double),
+                ("render_fraction", ctypes.c_double), ("swap_interval_s", ctypes.c_double),
+                ("query_min", ctypes.c_uint32), ("query_max", ctypes.c_uint32), ("render_w", ctypes.c_uint32),
+                ("render_h", ctypes.c_uint32), ("render_bounces", ctypes.c_uint32), ("swap_triangles", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64), ("threads", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("stats", ServerStats), ("wall_s", ctypes.c_double), ("requests", ctypes.c_uint64),
+                ("queries", ctypes.c_uint64), ("renders", ctypes.c_uint64), ("swaps", ctypes.c_uint64)]
+
+
+SERVE_BATCHED, SERVE_FIFO = 0, 1
+
+
 class WScene(ctypes.Structure):
     _fields_ = [("triangles", ctypes.c_void_p), ("num_triangles", ctypes.c_uint32),
                 ("materials", ctypes.c_void_p), ("num_materials", ctypes.c_uint32),
@@ -122,6 +153,9 @@ EXPORTS = [
     "rt_render_last_rays", "rt_bvh_build", "rt_bvh_build_seeded", "rt_scene_lights",
     "rt_batch_create", "rt_batch_set_config", "rt_batch_get_config", "rt_batch_launch", "rt_batch_autotune",
     "rt_batch_destroy",
+    "rt_server_create", "rt_server_destroy", "rt_server_scene_add", "rt_server_scene_remove", "rt_server_scene_list",
+    "rt_server_submit_query", "rt_server_submit_render", "rt_server_wait", "rt_server_drain", "rt_server_stats_get",
+    "rt_server_stats_reset", "rt_server_last_error", "rtw_serve_poisson",
     "rtw_scene_soup", "rtw_scene_icosphere", "rtw_scene_clusters", "rtw_scene_free", "rtw_rays_camera_tiles",
     "rtw_rays_shell", "rtw_rays_box", "rtw_log_uniform",
 ]
@@ -158,6 +192,19 @@ def lib():
         "rt_batch_launch": (i32, [P, P, P]),
         "rt_batch_autotune": (i32, [P, P, u32, ctypes.c_char_p, ctypes.POINTER(BatchConfig)]),
         "rt_batch_destroy": (i32, [P, P]),
+        "rt_server_create": (i32, [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
+        "rt_server_destroy": (i32, [P]),
+        "rt_server_scene_add": (i32, [P, ctypes.POINTER(SceneDesc), ctypes.POINTER(u64)]),
+        "rt_server_scene_remove": (i32, [P, u64]),
+        "rt_server_scene_list": (i32, [P, P, u32, ctypes.POINTER(u32)]),
+        "rt_server_submit_query": (i32, [P, u64, P, u32, P, P]),
+        "rt_server_submit_render": (i32, [P, u64, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), u32, u32, P, P]),
+        "rt_server_wait": (i32, [P, u64, i32, ctypes.POINTER(i32)]),
+        "rt_server_drain": (i32, [P]),
+        "rt_server_stats_get": (i32, [P, ctypes.POINTER(ServerStats)]),
+        "rt_server_stats_reset": (i32, [P]),
+        "rt_server_last_error": (ctypes.c_char_p, [P]),
+        "rtw_serve_poisson": (i32, [P, ctypes.POINTER(ServeParams), ctypes.POINTER(ServeResult)]),
         "rt_bvh_build": (i32, [P, u32, P]),
         "rt_bvh_build_seeded": (i32, [P, u32, u32, u32, P]),
         "rt_scene_lights": (i32, [P, u32, P, u32, P, u32, ctypes.POINTER(u32)]),
@@ -456,6 +503,91 @@ class Batch:
         if self.handle:
             self.ctx.check(lib().rt_batch_destroy(self.ctx.handle, self.handle), "rt_batch_destroy")
             self.handle = None
+
+
+class Server:
+    """An rt_server: resident scenes and a request loop on its own thread."""
+
+    def __init__(self, ctx: "Context", mode=SERVE_BATCHED, slots=0, max_batch_rays=0, max_render_pixels=0,
+                 hbm_budget=0):
+        self.ctx = ctx
+        cfg = ServerConfig(mode, slots, max_batch_rays, max_render_pixels, hbm_budget)
+        h = ctypes.c_void_p()
+        ctx.check(lib().rt_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)), "rt_server_create")
+        self.handle = h
+        self._keep = {}
+
+    def _check(self, rc, what):
+        if rc != RT_OK:
+            raise RtError(rc, f"{what}: {lib().rt_server_last_error(self.handle).decode()}")
+
+    def add_scene(self, hs: HostScene) -> int:
+        d = hs.desc()
+        sid = ctypes.c_uint64()
+        self._check(lib().rt_server_scene_add(self.handle, ctypes.byref(d), ctypes.byref(sid)), "rt_server_scene_add")
+        return sid.value
+
+    def remove_scene(self, sid: int):
+        self._check(lib().rt_server_scene_remove(self.handle, sid), "rt_server_scene_remove")
+
+    def scenes(self):
+        cnt = ctypes.c_uint32()
+        lib().rt_server_scene_list(self.handle, None, 0, ctypes.byref(cnt))
+        ids = (ctypes.c_uint64 * max(1, cnt.value))()
+        lib().rt_server_scene_list(self.handle, ids, cnt.value, ctypes.byref(cnt))
+        return [ids[i] for i in range(cnt.value)]
+
+    def submit_query(self, sid: int, rays: np.ndarray, hits: np.ndarray) -> int:
+        rid = ctypes.c_uint64()
+        self._check(lib().rt_server_submit_query(self.handle, sid, _np_ptr(rays), len(rays), _np_ptr(hits),
+                                                 ctypes.byref(rid)), "rt_server_submit_query")
+        self._keep[rid.value] = (rays, hits)
+        return rid.value
+
+    def submit_render(self, sid: int, cam_pos, w, h, target: np.ndarray, current_sample=0, vfov=60.0, bounces=4):
+        ubo = make_ubo(cam_pos, current_sample)
+        prm = RenderParams(vfov, bounces, 0, 0, 0)
+        rid = ctypes.c_uint64()
+        self._check(lib().rt_server_submit_render(self.handle, sid, ctypes.byref(ubo), ctypes.byref(prm), w, h,
+                                                  target.ctypes.data, ctypes.byref(rid)), "rt_server_submit_render")
+        self._keep[rid.value] = (target,)
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> int:
+        st = ctypes.c_int()
+        self._check(lib().rt_server_wait(self.handle, rid, timeout_ms, ctypes.byref(st)), "rt_server_wait")
+        self._keep.pop(rid, None)
+        return st.value
+
+    def drain(self):
+        self._check(lib().rt_server_drain(self.handle), "rt_server_drain")
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        self._check(lib().rt_server_stats_get(self.handle, ctypes.byref(s)), "rt_server_stats_get")
+        return s
+
+    def reset_stats(self):
+        lib().rt_server_stats_reset(self.handle)
+
+    def poisson(self, rate, duration, render_fraction=0.1, swap_interval=10.0, query_min=1000, query_max=262144,
+                render_w=512, render_h=512, render_bounces=4, swap_triangles=500000, seed=7, threads=16) -> ServeResult:
+        p = ServeParams(rate, duration, render_fraction, swap_interval, query_min, query_max, render_w, render_h,
+                        render_bounces, swap_triangles, seed, threads, 0)
+        r = ServeResult()
+        self._check(lib().rtw_serve_poisson(self.handle, ctypes.byref(p), ctypes.byref(r)), "rtw_serve_poisson")
+        return r
+
+    def close(self):
+        if self.handle:
+            lib().rt_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Scene:
