@@ -277,7 +277,7 @@ int rt_render_last_rays(const rt_ctx* ctx, uint64_t* primary, uint64_t* secondar
 enum { RT_SERVE_BATCHED = 0, RT_SERVE_FIFO = 1 };
 typedef struct rt_server_config {
     uint32_t mode;            /* RT_SERVE_*                                     */
-    uint32_t slots;           /* concurrent device launches (1..8, 0 = 3)       */
+    uint32_t slots;           /* concurrent device launches (1..8, 0 = 4)       */
     uint32_t max_batch_rays;  /* rays per batched launch (0 = 1,048,576)        */
     uint32_t max_render_pixels; /* largest render request (0 = 1,048,576)       */
     uint64_t hbm_budget;      /* bytes for resident scenes (0 = no limit)       */
@@ -293,6 +293,11 @@ typedef struct rt_server_stats {
 } rt_server_stats;
 
 typedef struct rt_server rt_server;
+/* Page-locked host memory for request buffers: rays, hits and targets in
+ * such memory are copied by DMA straight to and from the device; pageable
+ * buffers go through the slot's staging copy. */
+int rt_host_alloc(size_t bytes, void** out);
+int rt_host_free(void* p);
 int rt_server_create(rt_ctx* ctx, const rt_server_config* config, rt_server** out);
 int rt_server_destroy(rt_server* server);
 /* Uploads a scene and makes it resident, evicting least recently used idle

@@ -19,8 +19,8 @@ def scenes(rt):
     return [rt.scene_clusters(n, 300 + i) for i, n in enumerate([20000, 70000, 150000])]
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_results_same_alone_and_in_mix(rt, gpu_ctx, scenes, mode):
+@pytest.mark.parametrize("mode,pinned", [(0, False), (1, False), (0, True)])
+def test_results_same_alone_and_in_mix(rt, gpu_ctx, scenes, mode, pinned):
     srv = rt.Server(gpu_ctx, mode=mode, slots=3, max_batch_rays=1 << 18, max_render_pixels=128 * 128)
     ids = [srv.add_scene(hs) for hs in scenes]
     alone = [gpu_ctx.upload(hs) for hs in scenes]
@@ -29,14 +29,26 @@ def test_results_same_alone_and_in_mix(rt, gpu_ctx, scenes, mode):
     for k in range(48):
         s = int(rng.integers(0, 3))
         if k % 6 == 5:
-            tgt = np.zeros((96, 128, 4), np.uint8)
-            reqs.append(("render", s, tgt, srv.submit_render(ids[s], CAM, 128, 96, tgt, bounces=4)))
+            tgt = rt.pinned_array((96, 128, 4), np.uint8) if pinned else np.zeros((96, 128, 4), np.uint8)
+            reqs.append(["render", s, tgt, None])
         else:
             n = int(rng.integers(1, 40000))
             rays = rt.rays_box(n, 500 + k, 4.0)
             hits = np.zeros(n, rt.HIT_DT)
-            reqs.append(("query", s, (rays, hits), srv.submit_query(ids[s], rays, hits)))
-    print(f"case: server mode {mode}, 48 requests", flush=True)
+            if pinned:
+                pr = rt.pinned_array((n,), rt.RAY_DT)
+                pr[...] = rays
+                rays = pr
+                hits = rt.pinned_array((n,), rt.HIT_DT)
+            reqs.append(["query", s, (rays, hits), None])
+    print(f"case: server mode {mode} pinned {pinned}, 48 requests", flush=True)
+    # inputs are built first so the submits arrive back to back
+    for r in reqs:
+        kind, s, data, _ = r
+        if kind == "render":
+            r[3] = srv.submit_render(ids[s], CAM, 128, 96, data, bounces=4)
+        else:
+            r[3] = srv.submit_query(ids[s], data[0], data[1])
     for kind, s, data, rid in reqs:
         assert srv.wait(rid, 60000) == rt.RT_OK
     for kind, s, data, rid in reqs:

@@ -52,8 +52,19 @@ struct Request {
     int status = RT_OK;
     bool done = false;
     bool detached = false;       // nobody waits: freed on completion
+    bool direct = false;         // caller buffers are page-locked: DMA in place
     uint32_t offset = 0;         // first ray in the slot's staging
 };
+
+// True when `p` is page-locked host memory the device can DMA from.
+bool pinned(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
 
 struct Slot {
     rt_ctx* ctx = nullptr;
@@ -152,17 +163,23 @@ int launch_slot(rt_server* S, Slot& sl) {
         Request* r = sl.reqs[0];
         int rc = rt_render(sl.ctx, r->scene, &r->ubo, &r->rp, sl.d_target, sl.d_accum, r->w, r->h, st);
         if (rc) return rc;
-        if (hipMemcpyAsync(sl.h_target, sl.d_target, (size_t)r->w * r->h * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        if (hipMemcpyAsync(r->direct ? r->target : sl.h_target, sl.d_target, (size_t)r->w * r->h * 4,
+                           hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(sl.h_counters, sl.ctx->d_counters, 64 * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
             return RT_ERR_HIP;
     } else if (S->cfg.mode == RT_SERVE_FIFO) {
         Request* r = sl.reqs[0];
-        std::memcpy(sl.h_rays, r->rays, (size_t)r->n * sizeof(rt_ray));
-        if (hipMemcpyAsync(sl.d_rays, sl.h_rays, (size_t)r->n * sizeof(rt_ray), hipMemcpyHostToDevice, st) != hipSuccess)
+        const rt_ray* src = r->rays;
+        if (!r->direct) {
+            std::memcpy(sl.h_rays, r->rays, (size_t)r->n * sizeof(rt_ray));
+            src = sl.h_rays;
+        }
+        if (hipMemcpyAsync(sl.d_rays, src, (size_t)r->n * sizeof(rt_ray), hipMemcpyHostToDevice, st) != hipSuccess)
             return RT_ERR_HIP;
         int rc = rt_closest_hit(sl.ctx, r->scene, sl.d_rays, r->n, sl.d_hits, nullptr, st);
         if (rc) return rc;
-        if (hipMemcpyAsync(sl.h_hits, sl.d_hits, (size_t)r->n * sizeof(rt_hit), hipMemcpyDeviceToHost, st) != hipSuccess)
+        if (hipMemcpyAsync(r->direct ? r->hits : sl.h_hits, sl.d_hits, (size_t)r->n * sizeof(rt_hit),
+                           hipMemcpyDeviceToHost, st) != hipSuccess)
             return RT_ERR_HIP;
     } else {
         uint32_t off = 0, need = 0;
@@ -170,7 +187,16 @@ int launch_slot(rt_server* S, Slot& sl) {
         for (uint32_t i = 0; i < ni; ++i) {
             Request* r = sl.reqs[i];
             r->offset = off;
-            std::memcpy(sl.h_rays + off, r->rays, (size_t)r->n * sizeof(rt_ray));
+            if (r->direct) {
+                if (hipMemcpyAsync(sl.d_rays + off, r->rays, (size_t)r->n * sizeof(rt_ray), hipMemcpyHostToDevice, st) !=
+                    hipSuccess)
+                    return RT_ERR_HIP;
+            } else {
+                std::memcpy(sl.h_rays + off, r->rays, (size_t)r->n * sizeof(rt_ray));
+                if (hipMemcpyAsync(sl.d_rays + off, sl.h_rays + off, (size_t)r->n * sizeof(rt_ray),
+                                   hipMemcpyHostToDevice, st) != hipSuccess)
+                    return RT_ERR_HIP;
+            }
             BatchEntry& e = sl.h_table[i];
             const rt_scene* sc = r->scene;
             e.nodes = sc->d_w4nodes;
@@ -191,16 +217,19 @@ int launch_slot(rt_server* S, Slot& sl) {
         const uint32_t chunks = (off + 63) / 64;
         for (uint32_t i = 0, c = 0; i < ni && c < chunks; ++i)
             while (c < chunks && (uint64_t)c * 64 < sl.h_offsets[i + 1]) sl.h_chunk[c++] = i;
-        if (hipMemcpyAsync(sl.d_rays, sl.h_rays, (size_t)off * sizeof(rt_ray), hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(sl.d_table, sl.h_table, ni * sizeof(BatchEntry), hipMemcpyHostToDevice, st) != hipSuccess ||
+        if (hipMemcpyAsync(sl.d_table, sl.h_table, ni * sizeof(BatchEntry), hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(sl.d_offsets, sl.h_offsets, (ni + 1) * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(sl.d_chunk, sl.h_chunk, std::max(chunks, 1u) * 4, hipMemcpyHostToDevice, st) != hipSuccess)
             return RT_ERR_HIP;
         BatchIo bio{sl.d_table, sl.d_offsets, sl.d_chunk, nullptr};
         int rc = launch_batch_dynamic(sl.ctx, bio, sl.d_cursor, off, need, 32u, st);
         if (rc) return rc;
-        if (hipMemcpyAsync(sl.h_hits, sl.d_hits, (size_t)off * sizeof(rt_hit), hipMemcpyDeviceToHost, st) != hipSuccess)
-            return RT_ERR_HIP;
+        for (Request* r : sl.reqs) {
+            rt_hit* dst = r->direct ? r->hits : sl.h_hits + r->offset;
+            if (hipMemcpyAsync(dst, sl.d_hits + r->offset, (size_t)r->n * sizeof(rt_hit), hipMemcpyDeviceToHost, st) !=
+                hipSuccess)
+                return RT_ERR_HIP;
+        }
     }
     if (hipEventRecord(sl.ev, st) != hipSuccess) return RT_ERR_HIP;
     return RT_OK;
@@ -227,12 +256,13 @@ void finish_request(rt_server* S, Request* r, int status) {
 void retire_slot(rt_server* S, Slot& sl) {
     if (sl.render) {
         Request* r = sl.reqs[0];
-        std::memcpy(r->target, sl.h_target, (size_t)r->w * r->h * 4);
+        if (!r->direct) std::memcpy(r->target, sl.h_target, (size_t)r->w * r->h * 4);
         uint64_t rays = (uint64_t)r->w * r->h;
         for (int i = 1; i < 64; ++i) rays += sl.h_counters[i];
         S->st.rays += rays;
     } else {
         for (Request* r : sl.reqs) {
+            if (r->direct) continue;
             const uint32_t off = S->cfg.mode == RT_SERVE_FIFO ? 0u : r->offset;
             std::memcpy(r->hits, sl.h_hits + off, (size_t)r->n * sizeof(rt_hit));
         }
@@ -321,12 +351,24 @@ int submit(rt_server* S, Request* r, uint64_t* request_id) {
 
 }  // namespace
 
+extern "C" int rt_host_alloc(size_t bytes, void** out) {
+    if (!out) return RT_ERR_INVALID;
+    *out = nullptr;
+    if (bytes == 0) return RT_OK;
+    return hipHostMalloc(out, bytes, hipHostMallocDefault) == hipSuccess ? RT_OK : RT_ERR_NOMEM;
+}
+
+extern "C" int rt_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+    return RT_OK;
+}
+
 extern "C" int rt_server_create(rt_ctx* ctx, const rt_server_config* config, rt_server** out) {
     if (!ctx || !out) return RT_ERR_INVALID;
     *out = nullptr;
     rt_server_config c = config ? *config : rt_server_config{};
     if (c.mode > RT_SERVE_FIFO) return set_error(ctx, RT_ERR_INVALID, "rt_server_create: unknown mode");
-    if (c.slots == 0) c.slots = 3;
+    if (c.slots == 0) c.slots = 4;
     if (c.slots > 8) return set_error(ctx, RT_ERR_INVALID, "rt_server_create: at most 8 slots");
     if (c.mode == RT_SERVE_FIFO) c.slots = 1;
     if (c.max_batch_rays == 0) c.max_batch_rays = 1u << 20;
@@ -454,6 +496,7 @@ extern "C" int rt_server_submit_query(rt_server* S, uint64_t scene_id, const rt_
     r->rays = rays;
     r->n = n;
     r->hits = hits;
+    r->direct = pinned(rays) && pinned(hits);
     return submit(S, r, request_id);
 }
 
@@ -472,6 +515,7 @@ extern "C" int rt_server_submit_render(rt_server* S, uint64_t scene_id, const rt
     r->w = w;
     r->h = h;
     r->target = target;
+    r->direct = pinned(target);
     return submit(S, r, request_id);
 }
 

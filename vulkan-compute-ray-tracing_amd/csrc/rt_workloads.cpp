@@ -120,6 +120,20 @@ double value_noise(double x, double y, double z, uint64_t seed) {
     return acc;
 }
 
+// Page-locked buffers released together.
+struct PinnedArena {
+    std::vector<void*> ptrs;
+    ~PinnedArena() {
+        for (void* q : ptrs) rt_host_free(q);
+    }
+    void* get(size_t bytes) {
+        void* q = nullptr;
+        if (rt_host_alloc(bytes, &q) != RT_OK) return nullptr;
+        ptrs.push_back(q);
+        return q;
+    }
+};
+
 }  // namespace
 }  // namespace rtamd
 
@@ -331,11 +345,19 @@ extern "C" int rtw_serve_poisson(rt_server* S, const rtw_serve_params* p, rtw_se
     std::memset(res, 0, sizeof *res);
     rt_server_stats_reset(S);
     SplitMix rng(p->seed);
-    std::vector<rt_ray> pool(p->query_max);
-    rtw_rays_box(p->query_max, p->seed ^ 0x5eedull, 4.0f, pool.data());
+    // request buffers in page-locked memory, as a serving client would use
     const int kRing = 32;
-    std::vector<std::vector<rt_hit>> hit_ring(kRing, std::vector<rt_hit>(p->query_max));
-    std::vector<std::vector<uint8_t>> tgt_ring(kRing, std::vector<uint8_t>((size_t)p->render_w * p->render_h * 4));
+    PinnedArena mem;
+    rt_ray* pool = (rt_ray*)mem.get((size_t)p->query_max * sizeof(rt_ray));
+    std::vector<rt_hit*> hit_ring(kRing);
+    std::vector<uint8_t*> tgt_ring(kRing);
+    for (int i = 0; i < kRing; ++i) {
+        hit_ring[i] = (rt_hit*)mem.get((size_t)p->query_max * sizeof(rt_hit));
+        tgt_ring[i] = (uint8_t*)mem.get((size_t)p->render_w * p->render_h * 4);
+        if (!hit_ring[i] || !tgt_ring[i]) return RT_ERR_NOMEM;
+    }
+    if (!pool) return RT_ERR_NOMEM;
+    rtw_rays_box(p->query_max, p->seed ^ 0x5eedull, 4.0f, pool);
 
     std::mutex ids_mu;
     std::vector<uint64_t> ids;
@@ -400,12 +422,11 @@ extern "C" int rtw_serve_poisson(rt_server* S, const rtw_serve_params* p, rtw_se
                 rt_render_params rp{};
                 rp.vfov_deg = 60.0f;
                 rp.bounces = p->render_bounces;
-                rc = rt_server_submit_render(S, id, &ubo, &rp, p->render_w, p->render_h,
-                                             tgt_ring[k % kRing].data(), nullptr);
+                rc = rt_server_submit_render(S, id, &ubo, &rp, p->render_w, p->render_h, tgt_ring[k % kRing], nullptr);
             } else {
                 uint32_t n = (uint32_t)std::floor(std::exp(un * lq) * (double)p->query_min);
                 n = std::max(p->query_min, std::min(p->query_max, n));
-                rc = rt_server_submit_query(S, id, pool.data(), n, hit_ring[k % kRing].data(), nullptr);
+                rc = rt_server_submit_query(S, id, pool, n, hit_ring[k % kRing], nullptr);
             }
             if (rc == RT_ERR_NOT_FOUND) { refresh(); continue; }   // the scene was just evicted
             if (rc == RT_OK) { res->requests++; (render ? res->renders : res->queries)++; }

@@ -153,7 +153,7 @@ EXPORTS = [
     "rt_render_last_rays", "rt_bvh_build", "rt_bvh_build_seeded", "rt_scene_lights",
     "rt_batch_create", "rt_batch_set_config", "rt_batch_get_config", "rt_batch_launch", "rt_batch_autotune",
     "rt_batch_destroy",
-    "rt_server_create", "rt_server_destroy", "rt_server_scene_add", "rt_server_scene_remove", "rt_server_scene_list",
+    "rt_host_alloc", "rt_host_free", "rt_server_create", "rt_server_destroy", "rt_server_scene_add", "rt_server_scene_remove", "rt_server_scene_list",
     "rt_server_submit_query", "rt_server_submit_render", "rt_server_wait", "rt_server_drain", "rt_server_stats_get",
     "rt_server_stats_reset", "rt_server_last_error", "rtw_serve_poisson",
     "rtw_scene_soup", "rtw_scene_icosphere", "rtw_scene_clusters", "rtw_scene_free", "rtw_rays_camera_tiles",
@@ -192,6 +192,8 @@ def lib():
         "rt_batch_launch": (i32, [P, P, P]),
         "rt_batch_autotune": (i32, [P, P, u32, ctypes.c_char_p, ctypes.POINTER(BatchConfig)]),
         "rt_batch_destroy": (i32, [P, P]),
+        "rt_host_alloc": (i32, [ctypes.c_size_t, ctypes.POINTER(P)]),
+        "rt_host_free": (i32, [P]),
         "rt_server_create": (i32, [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
         "rt_server_destroy": (i32, [P]),
         "rt_server_scene_add": (i32, [P, ctypes.POINTER(SceneDesc), ctypes.POINTER(u64)]),
@@ -265,6 +267,20 @@ def _take(ws: WScene) -> HostScene:
                          arr(ws.spheres, ws.num_spheres, SPHERE_DT))
     finally:
         lib().rtw_scene_free(ctypes.byref(ws))
+
+
+def pinned_array(shape, dtype) -> np.ndarray:
+    """A numpy array in page-locked host memory (rt_host_alloc), freed with the array."""
+    import weakref
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape))
+    nbytes = max(1, count * dt.itemsize)
+    ptr = ctypes.c_void_p()
+    _check(lib().rt_host_alloc(nbytes, ctypes.byref(ptr)), what="rt_host_alloc")
+    buf = (ctypes.c_char * nbytes).from_address(ptr.value)
+    arr = np.frombuffer(buf, dtype=np.uint8, count=count * dt.itemsize).view(dt).reshape(shape)
+    weakref.finalize(buf, lib().rt_host_free, ptr.value)
+    return arr
 
 
 def scene_soup(n: int, seed: int, threads: int = 8) -> HostScene:
