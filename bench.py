@@ -13,8 +13,19 @@ records x 64 B (4-wide; 32 B with --layout q16) + triangle records x 48 B + ray 
 hit 16 B, counted per ray by an untimed pass of the counting kernel), the
 CPU baseline (the
 oracle's restatement of hit_bvh in the shader's visiting order, on a bounded
-sample of the same rays), and side measurements of the 1920x1080 x 4 spp,
-8-bounce render of the same scene.
+sample of the same rays), and side measurements:
+  render   the 1920x1080 x 4 spp, 8-bounce render of the same scene;
+  batched  64 clustered-soup scenes (triangles log-uniform in [1k, 1M], seeds
+           100+i) with ray batches log-uniform in [4k, 2M] (seeds 200+i):
+           64 plain launches against one autotuned persistent launch
+           replayed from a hipGraph;
+  served   16 resident scenes (log-uniform [10k, 2M], seeds 300-315) under
+           an HBM budget of 1.05x their footprint, Poisson(4000/s) requests
+           (10% 512x512 1-spp 4-bounce renders, the rest queries of
+           log-uniform [1k, 256k] rays), a new 500k-triangle scene every 10 s:
+           p50/p99 latency and Mrays/s of the batching server, against the
+           plain kernel serving the first seconds of the same trace FIFO.
+Progress lines go to stderr; the JSON line is the only stdout.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -48,17 +59,127 @@ def read_traffic():
     return None
 
 
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def median_ms(fn, reps: int) -> float:
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ts = []
+    for _ in range(reps):
+        s.record()
+        fn()
+        e.record()
+        torch.cuda.synchronize()
+        ts.append(s.elapsed_time(e))
+    return sorted(ts)[len(ts) // 2]
+
+
+def bench_batched(ctx, threads: int, reps: int):
+    """64 scenes x uneven ray batches: plain launches vs one batched launch."""
+    import torch
+    import rtamd
+    t0 = time.perf_counter()
+    scenes, plain_items, batch_items = [], [], []
+    for i in range(64):
+        hs = rtamd.scene_clusters(rtamd.log_uniform(100 + i, 1000, 1000000), 100 + i, threads=threads)
+        sc = ctx.upload(hs)
+        c = rtamd.log_uniform(200 + i, 4096, 2097152)
+        rays = rtamd.rays_to_device(rtamd.rays_box(c, 200 + i, 4.0))
+        scenes.append(sc)
+        plain_items.append((sc, rays, torch.empty((c, 4), dtype=torch.float32, device="cuda")))
+        batch_items.append((sc, rays, torch.empty((c, 4), dtype=torch.float32, device="cuda")))
+    tris = sum(s.info.num_triangles for s in scenes)
+    nrays = sum(r.shape[0] for _, r, _ in plain_items)
+    log(f"batched: 64 scenes, {tris} triangles, {nrays} rays built in {time.perf_counter() - t0:.1f} s")
+
+    def plain():
+        for sc, r, h in plain_items:
+            ctx.closest_hit(sc, r, h)
+
+    t_plain = median_ms(plain, reps)
+    b = rtamd.Batch(ctx, batch_items)
+    t0 = time.perf_counter()
+    best = b.autotune(trials=3)
+    t_tune = time.perf_counter() - t0
+    t_batch = median_ms(b.launch, reps)
+    torch.cuda.synchronize()
+    same = all(torch.equal(hp.view(torch.int32), hb.view(torch.int32))
+               for (_, _, hp), (_, _, hb) in zip(plain_items, batch_items))
+    log(f"batched: plain {t_plain:.2f} ms, batched {t_batch:.2f} ms, ratio {t_plain / t_batch:.2f}")
+    out = {"workload": "64 clustered-soup scenes, triangles log-uniform [1k,1M] (seeds 100+i), ray batches "
+                       "log-uniform [4k,2M] (seeds 200+i, origins U[-4,4]^3, uniform directions)",
+           "scenes": 64, "triangles": tris, "rays": nrays,
+           "plain_ms": round(t_plain, 3), "plain_mrays_per_s": round(nrays / t_plain / 1e3, 1),
+           "batched_ms": round(t_batch, 3), "mrays_per_s": round(nrays / t_batch / 1e3, 1),
+           "ratio": round(t_plain / t_batch, 3), "hits_identical": bool(same),
+           "autotune_s": round(t_tune, 1), "config": best.as_dict()}
+    b.destroy()
+    del b, plain_items, batch_items, scenes
+    torch.cuda.synchronize()
+    return out
+
+
+def bench_served(ctx, threads: int, seconds: float, fifo_seconds: float):
+    """Poisson request stream against the batching server and the FIFO plain server."""
+    import rtamd
+    t0 = time.perf_counter()
+    hss = [rtamd.scene_clusters(rtamd.log_uniform(300 + i, 10000, 2000000), 300 + i, threads=threads)
+           for i in range(16)]
+    tris = sum(len(h.triangles) for h in hss)
+    log(f"served: 16 scenes, {tris} triangles built in {time.perf_counter() - t0:.1f} s")
+    probe = rtamd.Server(ctx)
+    for h in hss:
+        probe.add_scene(h)
+    budget = int(probe.stats().resident_bytes * 1.05)
+    probe.close()
+
+    def run(mode, slots, duration, warm):
+        srv = rtamd.Server(ctx, mode=mode, slots=slots, hbm_budget=budget)
+        for h in hss:
+            srv.add_scene(h)
+        if warm:
+            srv.poisson(rate=4000.0, duration=1.0, swap_interval=0.0, seed=11, threads=threads)
+        r = srv.poisson(rate=4000.0, duration=duration, render_fraction=0.1, swap_interval=10.0,
+                        swap_triangles=500000, seed=7, threads=threads)
+        srv.close()
+        st = r.stats
+        return {"requests": int(r.requests), "queries": int(r.queries), "renders": int(r.renders),
+                "completed": int(st.completed), "failed": int(st.failed), "launches": int(st.launches),
+                "scenes_added": int(st.scenes_added), "scenes_evicted": int(st.scenes_evicted),
+                "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3), "mean_ms": round(st.mean_ms, 3),
+                "max_ms": round(st.max_ms, 3), "rays": int(st.rays), "wall_s": round(r.wall_s, 2),
+                "mrays_per_s": round(st.rays / r.wall_s / 1e6, 1)}
+
+    served = run(rtamd.SERVE_BATCHED, 0, seconds, True)
+    log(f"served: p50 {served['p50_ms']} ms p99 {served['p99_ms']} ms, {served['mrays_per_s']} Mrays/s")
+    fifo = run(rtamd.SERVE_FIFO, 1, fifo_seconds, False)
+    log(f"served: fifo p50 {fifo['p50_ms']} ms, {fifo['mrays_per_s']} Mrays/s")
+    out = {"workload": "16 resident clustered-soup scenes, triangles log-uniform [10k,2M] (seeds 300-315), "
+                       "HBM budget 1.05x their footprint, Poisson(4000/s) arrivals for "
+                       f"{seconds:g} s: 10% 512x512 1-spp 4-bounce renders, 90% closest-hit queries of "
+                       "log-uniform [1k,256k] rays, a new 500k-triangle scene every 10 s (LRU eviction)",
+           "triangles": tris, "hbm_budget_bytes": budget, **served,
+           "fifo": {"sample": f"first {fifo_seconds:g} s of the same arrival sequence, plain kernel one "
+                              f"request at a time", **fifo},
+           "ratio": round(served["mrays_per_s"] / max(fifo["mrays_per_s"], 1e-9), 3)}
+    return out
+
+
 def cpu_baseline(hs, rays_np, seconds: float, threads: int):
     import oracle as O
     osc = O.Scene(hs.triangles, hs.materials, hs.nodes, hs.lights, hs.spheres)
     m = 64 * threads
-    t0 = time.perf_counter()
-    O.closest_hit(osc, rays_np[:m], O.TRAV_REFERENCE, threads=threads)
-    dt = time.perf_counter() - t0
-    m = int(min(len(rays_np), max(m, m * seconds / max(dt, 1e-3))))
-    t0 = time.perf_counter()
-    O.closest_hit(osc, rays_np[:m], O.TRAV_REFERENCE, threads=threads)
-    dt = time.perf_counter() - t0
+    for _ in range(6):
+        t0 = time.perf_counter()
+        O.closest_hit(osc, rays_np[:m], O.TRAV_REFERENCE, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 * seconds or m == len(rays_np):
+            break
+        m = int(min(len(rays_np), max(2 * m, m * seconds / max(dt, 1e-3))))
     return {"value": round(m / dt / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"first {m} of the {len(rays_np)} incoherent rays, oracle hit_bvh in the shader's "
                       f"visiting order (line test, both children pushed), {threads} threads, {dt:.1f} s"}
@@ -74,6 +195,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--fifo-seconds", type=float, default=10.0)
     ap.add_argument("--layout", choices=["w4", "q16", "f32"], default="w4",
                     help="node records: 4-wide 64-byte 16-bit (default), binary 32-byte 16-bit, binary 64-byte fp32")
     ap.add_argument("--refill", type=int, default=0, help="idle lanes before a wave refetches (0 = library default)")
@@ -94,7 +219,9 @@ def main():
     layout = {"w4": rtamd.LAYOUT_W4, "q16": rtamd.LAYOUT_Q16, "f32": rtamd.LAYOUT_F32}[args.layout]
     node_bytes = 32 if layout == rtamd.LAYOUT_Q16 else 64
     ctx.set_tuning(node_layout=layout, refill=args.refill)
+    t0 = time.perf_counter()
     hs = rtamd.scene_clusters(args.tris, 3, threads=threads)
+    log(f"scene {len(hs.triangles)} triangles built in {time.perf_counter() - t0:.1f} s")
     scene = ctx.upload(hs)
     rays_np = rtamd.rays_box(args.rays, 4, 4.0)
     rays = rtamd.rays_to_device(rays_np)
@@ -121,6 +248,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    log(f"plain: {ms_per_step:.3f} ms per step")
     kernel_ms = sum(a.elapsed_time(b) for a, b in evs) / args.steps
     value = n / (ms_per_step / 1e3) / 1e6
     achieved = bytes_per_launch / (kernel_ms / 1e3) / 1e9
@@ -156,6 +284,11 @@ def main():
         out["render"] = {"workload": "1920x1080, 4 spp (currentSample 0..3), 8 bounces, vfov 60, eye (0,0,10)",
                          "ms": round(rms, 3), "rays": prim + sec, "secondary_rays": sec,
                          "mrays_per_s": round((prim + sec) / (rms / 1e3) / 1e6, 2)}
+
+    if not args.no_batched:
+        out["batched"] = bench_batched(ctx, threads, 5)
+    if not args.no_served:
+        out["served"] = bench_served(ctx, threads, args.serve_seconds, args.fifo_seconds)
 
     if args.copy_probe_mib:
         src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
