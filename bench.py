@@ -41,6 +41,7 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "vulkan-compute-ray-tracing_amd")]
 METRIC = "incoherent Mrays/s + p50/p99 latency; batched/served vs plain kernel, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_packed_clusters4M_traffic.txt")
+CANON_SAMPLE = 65536
 
 
 def host_threads() -> int:
@@ -180,9 +181,68 @@ def cpu_baseline(hs, rays_np, seconds: float, threads: int):
         if dt >= 0.5 * seconds or m == len(rays_np):
             break
         m = int(min(len(rays_np), max(2 * m, m * seconds / max(dt, 1e-3))))
+    k = min(CANON_SAMPLE, len(rays_np))
+    _, nv, tt, _ = O.closest_hit(osc, rays_np[:k], O.TRAV_ORDERED, stats=True, threads=threads)
     return {"value": round(m / dt / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"first {m} of the {len(rays_np)} incoherent rays, oracle hit_bvh in the shader's "
-                      f"visiting order (line test, both children pushed), {threads} threads, {dt:.1f} s"}
+                      f"visiting order (line test, both children pushed), {threads} threads, {dt:.1f} s; "
+                      "CPU restatement, not lavapipe",
+            "cpu_model": cpu_model(),
+            "canonical": {"rays": k, "traversal": "oracle ordered, t-culled (nearer child first)",
+                          "node_visits_per_ray": round(float(nv.mean()), 3),
+                          "tri_tests_per_ray": round(float(tt.mean()), 3)}}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def copy_bandwidth(mib: int = 1024) -> float:
+    """Device-to-device copy of `mib` MiB: (read + written bytes) / time, GB/s."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    ms = median_ms(lambda: dst.copy_(src), 5)
+    del src, dst
+    return 2.0 * (mib << 20) / (ms / 1e3) / 1e9
+
+
+def bench_small_and_medium(ctx, threads: int):
+    """The reference's own small render (1,024-triangle soup) and the
+    81,920-triangle icosphere query (coherent and incoherent halves)."""
+    import torch
+    import rtamd
+    out = {}
+    hs = rtamd.scene_soup(1024, 1, threads=threads)
+    sc = ctx.upload(hs)
+    tgt = torch.zeros((256, 256, 4), dtype=torch.uint8, device="cuda")
+    acc = torch.zeros_like(tgt)
+    ms = median_ms(lambda: ctx.render(sc, [0.0, 3.0, 0.0], 256, 256, vfov=60.0, bounces=4, target=tgt,
+                                      accum=acc), 7)
+    prim, sec = ctx.last_rays()
+    out["small_render"] = {"workload": "1024-triangle soup (seed 1), 256x256, 1 spp, 4 bounces, vfov 60, "
+                                       "eye (0,0,3)", "ms": round(ms, 4), "rays": prim + sec,
+                           "mrays_per_s": round((prim + sec) / ms / 1e3, 2)}
+    ico = rtamd.scene_icosphere(6, 2, 0.1, 4.0, threads=threads)
+    isc = ctx.upload(ico)
+    coh = rtamd.rays_to_device(rtamd.rays_camera_tiles([0, 3, 0], 60.0, 1024, 1024, 8, 0, 524288))
+    inc = rtamd.rays_to_device(rtamd.rays_shell(524288, 2024, 3.0, 0.5))
+    hits = torch.empty((524288, 4), dtype=torch.float32, device="cuda")
+    t_coh = median_ms(lambda: ctx.closest_hit(isc, coh, hits), 7)
+    t_inc = median_ms(lambda: ctx.closest_hit(isc, inc, hits), 7)
+    out["query_medium"] = {"workload": "level-6 icosphere (81920 triangles, displacement 0.1 value noise seed 2, "
+                                       "frequency 4); 524288 coherent camera rays (8x8 tiles of 1024x1024, eye "
+                                       "(0,0,3), vfov 60) and 524288 incoherent shell rays (seed 2024)",
+                           "coherent_ms": round(t_coh, 4), "coherent_mrays_per_s": round(524288 / t_coh / 1e3, 1),
+                           "incoherent_ms": round(t_inc, 4),
+                           "incoherent_mrays_per_s": round(524288 / t_inc / 1e3, 1)}
+    return out
 
 
 def main():
@@ -236,6 +296,16 @@ def main():
     bytes_per_launch = v_node * node_bytes + v_tri * 48 + n * (32 + 16)
     del nv, tt
 
+    # the same counts on the canonical sample, for the binary fp32 records
+    # (one record per internal node expanded) and the configured layout
+    k = min(CANON_SAMPLE, n)
+    sample_counts = {}
+    for name, lay in (("f32", rtamd.LAYOUT_F32), (args.layout, layout)):
+        ctx.set_tuning(node_layout=lay, refill=args.refill)
+        _, snv, stt = ctx.closest_hit_stats(scene, rays[:k])
+        sample_counts[name] = (float(snv.to(torch.float64).mean().item()), float(stt.to(torch.float64).mean().item()))
+    ctx.set_tuning(node_layout=layout, refill=args.refill)
+
     for _ in range(args.warmup):
         ctx.closest_hit(scene, rays, hits)
     torch.cuda.synchronize()
@@ -253,6 +323,7 @@ def main():
     value = n / (ms_per_step / 1e3) / 1e6
     achieved = bytes_per_launch / (kernel_ms / 1e3) / 1e9
     traffic = read_traffic()
+    copy_gbs = copy_bandwidth()
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -265,7 +336,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_trace_packed", "kernel_ms": round(kernel_ms, 4),
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "stream_copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
         "visits": {"node_records_per_ray": round(v_node / n, 2), "tri_records_per_ray": round(v_tri / n, 3)},
     }
 
@@ -281,9 +353,21 @@ def main():
         torch.cuda.synchronize()
         rms = e0.elapsed_time(e1)
         prim, sec = ctx.last_rays()
+        ctx.set_render_timing(True)
+        ctx.render_frames(scene, [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8, target=tgt, accum=acc)
+        st = ctx.last_render_stats()
+        ctx.set_render_timing(False)
         out["render"] = {"workload": "1920x1080, 4 spp (currentSample 0..3), 8 bounces, vfov 60, eye (0,0,10)",
                          "ms": round(rms, 3), "rays": prim + sec, "secondary_rays": sec,
-                         "mrays_per_s": round((prim + sec) / (rms / 1e3) / 1e6, 2)}
+                         "mrays_per_s": round((prim + sec) / (rms / 1e3) / 1e6, 2),
+                         "trace_primary_ms": round(st.trace_primary_ms, 3),
+                         "trace_secondary_ms": round(st.trace_secondary_ms, 3),
+                         "shade_ms": round(st.shade_ms, 3),
+                         "primary_trace_mrays_per_s": round(prim / st.trace_primary_ms / 1e3, 1),
+                         "secondary_trace_mrays_per_s": round(sec / st.trace_secondary_ms / 1e3, 1)}
+        log(f"render: {rms:.1f} ms, secondary traversal {out['render']['secondary_trace_mrays_per_s']} Mrays/s")
+
+    out.update(bench_small_and_medium(ctx, threads))
 
     if not args.no_batched:
         out["batched"] = bench_batched(ctx, threads, 5)
@@ -301,6 +385,24 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hs, rays_np, args.cpu_seconds, threads)
+        canon = out["cpu_baseline"]["canonical"]
+        f32v, f32t = sample_counts["f32"]
+        lv, lt = sample_counts[args.layout]
+        # wasted-work guard: the kernel's binary traversal against the
+        # canonical ordered one (visits), and the bytes per ray that
+        # `achieved` counts against the canonical visits priced at the most
+        # compact binary record (32 B) + 48 B triangles + 48 B ray and hit
+        canon_bytes = canon["node_visits_per_ray"] * 32 + canon["tri_tests_per_ray"] * 48 + 48
+        layout_bytes = lv * node_bytes + lt * 48 + 48
+        out["roofline"]["visit_guard"] = {
+            "rays": canon["rays"],
+            "f32_node_visits_per_ray": round(f32v, 3), "f32_tri_tests_per_ray": round(f32t, 3),
+            "f32_visits_vs_canonical": round(f32v / canon["node_visits_per_ray"], 4),
+            f"{args.layout}_records_per_ray": round(lv, 3), f"{args.layout}_tri_tests_per_ray": round(lt, 3),
+            f"{args.layout}_tri_tests_vs_canonical": round(lt / canon["tri_tests_per_ray"], 4),
+            "canonical_bytes_per_ray": round(canon_bytes, 1), f"{args.layout}_bytes_per_ray": round(layout_bytes, 1),
+            "bytes_vs_canonical": round(layout_bytes / canon_bytes, 4),
+            "credited": bool(f32v <= 1.05 * canon["node_visits_per_ray"] and layout_bytes <= 1.05 * canon_bytes)}
     print(json.dumps(out), flush=True)
 
 

@@ -263,6 +263,21 @@ int rt_render_frames(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo,
  * (one count per hit_bvh call, i.e. per ray). */
 int rt_render_last_rays(const rt_ctx* ctx, uint64_t* primary, uint64_t* secondary);
 
+/* Stage timing of renders (no counterpart in the reference, whose frame
+ * loop has no timers).  With timing on, each render records HIP events
+ * around every bounce's traversal kernel; rt_render_last_stats waits for
+ * the last render and splits its time into the bounce-0 traversal, the
+ * traversals of bounces 1.. (all secondary, incoherent rays) and the rest
+ * (generate, shade, store).  Without timing the *_ms fields are 0 and
+ * `timed` is 0. */
+typedef struct rt_render_stats {
+    uint64_t primary_rays, secondary_rays;
+    uint32_t bounces, timed;
+    float total_ms, trace_primary_ms, trace_secondary_ms, shade_ms;
+} rt_render_stats;
+int rt_render_set_timing(rt_ctx* ctx, int enable);
+int rt_render_last_stats(rt_ctx* ctx, rt_render_stats* out);
+
 /* ------------------------------------------------------------------ served */
 /* A request loop on its own thread: scenes stay resident under an HBM
  * budget (least recently used scenes without pending work are evicted to

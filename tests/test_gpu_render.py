@@ -107,3 +107,25 @@ def test_render_is_deterministic(rt, gpu_ctx, soup):
     a = gpu_ctx.render(scene, CAM, 192, 160, vfov=60.0, bounces=4).cpu().numpy()
     b = gpu_ctx.render(scene, CAM, 192, 160, vfov=60.0, bounces=4).cpu().numpy()
     assert np.array_equal(a, b)
+
+
+def test_stage_timing_leaves_the_image_unchanged(rt, gpu_ctx, soup):
+    """rt_render_set_timing only adds events: same image, same ray counts,
+    and the stage times add up to the whole render."""
+    hs, scene = soup
+    print("case: timed render 160x128, 4 bounces", flush=True)
+    plain = gpu_ctx.render(scene, CAM, 160, 128, vfov=60.0, bounces=4).cpu().numpy()
+    st0 = gpu_ctx.last_render_stats()
+    assert st0.timed == 0 and st0.total_ms == 0.0
+    gpu_ctx.set_render_timing(True)
+    try:
+        timed = gpu_ctx.render(scene, CAM, 160, 128, vfov=60.0, bounces=4).cpu().numpy()
+        st = gpu_ctx.last_render_stats()
+    finally:
+        gpu_ctx.set_render_timing(False)
+    assert np.array_equal(plain, timed)
+    assert st.timed == 1 and st.bounces == 4
+    assert (st.primary_rays, st.secondary_rays) == (st0.primary_rays, st0.secondary_rays)
+    assert st.primary_rays == 160 * 128 and st.secondary_rays > 0
+    assert st.trace_primary_ms > 0 and st.trace_secondary_ms > 0 and st.shade_ms >= 0
+    assert st.trace_primary_ms + st.trace_secondary_ms <= st.total_ms * (1 + 1e-5)

@@ -86,6 +86,8 @@ extern "C" int rt_destroy(rt_ctx* ctx) {
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_cursors) (void)hipFree(ctx->d_cursors);
+    for (hipEvent_t e : ctx->render_ev)
+        if (e) (void)hipEventDestroy(e);
     delete ctx;
     return RT_OK;
 }
@@ -169,11 +171,14 @@ int render_common(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo, const r
     if (!(params->vfov_deg > 0.0f && params->vfov_deg < 180.0f))
         return set_error(ctx, RT_ERR_INVALID, "render: vfov must lie in (0, 180) degrees");
     RT_HIP(ctx, hipSetDevice(ctx->device));
+    ctx->last_timed = false;
     int rc = render_impl(ctx, scene, *ubo, *params, frames, (uint32_t*)target, (const uint32_t*)accum_in,
                          (uint32_t*)accum_out, w, h, (hipStream_t)stream);
     if (rc) return rc;
     ctx->last_primary = (uint64_t)w * h * frames;
     ctx->last_secondary = UINT64_MAX;   // read lazily from the counters
+    ctx->last_bounces = params->bounces;
+    ctx->last_timed = ctx->time_render && params->bounces > 0;
     return RT_OK;
 }
 
@@ -188,6 +193,34 @@ extern "C" int rt_render_frames(rt_ctx* ctx, const rt_scene* scene, const rt_ubo
                                 uint32_t frames, uint8_t* target_dev, uint8_t* accum_dev, uint32_t w, uint32_t h,
                                 void* stream) {
     return render_common(ctx, scene, ubo, params, frames, target_dev, accum_dev, accum_dev, w, h, stream);
+}
+
+extern "C" int rt_render_set_timing(rt_ctx* ctx, int enable) {
+    if (!ctx) return RT_ERR_INVALID;
+    ctx->time_render = enable != 0;
+    return RT_OK;
+}
+
+extern "C" int rt_render_last_stats(rt_ctx* ctx, rt_render_stats* out) {
+    if (!ctx || !out) return RT_ERR_INVALID;
+    std::memset(out, 0, sizeof *out);
+    int rc = rt_render_last_rays(ctx, &out->primary_rays, &out->secondary_rays);
+    if (rc) return rc;
+    out->bounces = ctx->last_bounces;
+    if (!ctx->last_timed) return RT_OK;
+    hipEvent_t* ev = ctx->render_ev;
+    const uint32_t B = ctx->last_bounces;
+    RT_HIP(ctx, hipEventSynchronize(ev[1 + 2 * B]));
+    auto ms = [&](hipEvent_t a, hipEvent_t b) -> float {
+        float t = 0.0f;
+        return hipEventElapsedTime(&t, a, b) == hipSuccess ? t : 0.0f;
+    };
+    out->total_ms = ms(ev[0], ev[1 + 2 * B]);
+    out->trace_primary_ms = ms(ev[1], ev[2]);
+    for (uint32_t b = 1; b < B; ++b) out->trace_secondary_ms += ms(ev[1 + 2 * b], ev[2 + 2 * b]);
+    out->shade_ms = out->total_ms - out->trace_primary_ms - out->trace_secondary_ms;
+    out->timed = 1;
+    return RT_OK;
 }
 
 extern "C" int rt_render_last_rays(const rt_ctx* cctx, uint64_t* primary, uint64_t* secondary) {

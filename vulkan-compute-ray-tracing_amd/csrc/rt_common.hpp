@@ -169,6 +169,13 @@ struct rt_ctx {
     uint32_t cursor_next = 0;
     rt_tuning tuning{};               // 0 fields = defaults
     uint64_t last_primary = 0, last_secondary = 0;
+    // opt-in stage timing of renders: events around generate, each bounce's
+    // trace and shade, and store (created on the first timed render)
+    static constexpr int kRenderEvents = 2 * 63 + 2;
+    bool time_render = false;
+    hipEvent_t render_ev[kRenderEvents] = {};
+    uint32_t last_bounces = 0;
+    bool last_timed = false;
 };
 
 namespace rtamd {

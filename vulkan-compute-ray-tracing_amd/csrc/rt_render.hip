@@ -305,6 +305,15 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
     rc = launch_fill_u32(ctx, counters, 0u, 64u, s);
     if (rc) return rc;
 
+    // timing mode: ev[0] before generate, ev[1+2b] / ev[2+2b] around bounce
+    // b's traversal, ev[1+2*bounces] after store
+    hipEvent_t* ev = ctx->time_render ? ctx->render_ev : nullptr;
+    if (ev) {
+        for (uint32_t i = 0; i < 2 * bounces + 2; ++i)
+            if (!ev[i]) RT_HIP(ctx, hipEventCreate(&ev[i]));
+        RT_HIP(ctx, hipEventRecord(ev[0], s));
+    }
+
     Camera cam = make_camera(ubo, prm.vfov_deg, w, h);
     const uint32_t cu_cap = (uint32_t)ctx->num_cus * 16u;
     hipLaunchKernelGGL(k_generate, dim3(grid_for(P, cu_cap)), dim3(256), 0, s, w, h, frames, ubo.currentSample, cam,
@@ -314,8 +323,10 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
     for (uint32_t b = 0; b < bounces; ++b) {
         const uint32_t* n_dev = b == 0 ? nullptr : counters + b;
         TraceArgs ta{sc, rays[b & 1], P, n_dev, hits, prm.spheres, nullptr, nullptr, true};
+        if (ev) RT_HIP(ctx, hipEventRecord(ev[1 + 2 * b], s));
         rc = use_packed ? launch_trace_packed(ctx, ta, s) : launch_trace_reference(ctx, ta, prm.max_stack, s);
         if (rc) return rc;
+        if (ev) RT_HIP(ctx, hipEventRecord(ev[2 + 2 * b], s));
         hipLaunchKernelGGL(k_shade, dim3(grid_for(P, cu_cap)), dim3(256), 0, s, rays[b & 1], pids[b & 1], n_dev, P,
                            hits, rays[(b + 1) & 1], pids[(b + 1) & 1], counters + b + 1, color, rng, sc->d_tris,
                            sc->d_mats, sc->d_lights, sc->info.num_lights, sc->d_spheres, sc->info.num_triangles,
@@ -325,6 +336,7 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
     hipLaunchKernelGGL(k_store, dim3(grid_for((uint64_t)w * h, cu_cap)), dim3(256), 0, s, w * h, frames,
                        ubo.currentSample, color, accum_in, target, accum_out);
     RT_HIP(ctx, hipGetLastError());
+    if (ev) RT_HIP(ctx, hipEventRecord(ev[1 + 2 * bounces], s));
     return RT_OK;
 }
 

@@ -98,6 +98,15 @@ def batch_config(block=256, stack_lds=0, regroup=0, refill=32, blocks_per_cu=0, 
     return BatchConfig(block, stack_lds, regroup, refill, blocks_per_cu, use_graph)
 
 
+class RenderStats(ctypes.Structure):
+    _fields_ = [("primary_rays", ctypes.c_uint64), ("secondary_rays", ctypes.c_uint64),
+                ("bounces", ctypes.c_uint32), ("timed", ctypes.c_uint32)] + \
+               [(k, ctypes.c_float) for k in ("total_ms", "trace_primary_ms", "trace_secondary_ms", "shade_ms")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("slots", ctypes.c_uint32), ("max_batch_rays", ctypes.c_uint32),
                 ("max_render_pixels", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
@@ -150,7 +159,7 @@ EXPORTS = [
     "rt_create", "rt_destroy", "rt_last_error", "rt_status_string", "rt_scene_upload", "rt_scene_info_get",
     "rt_scene_check", "rt_tuning_set", "rt_tuning_get",
     "rt_scene_release", "rt_closest_hit", "rt_closest_hit_stats", "rt_render", "rt_render_frames",
-    "rt_render_last_rays", "rt_bvh_build", "rt_bvh_build_seeded", "rt_scene_lights",
+    "rt_render_last_rays", "rt_render_set_timing", "rt_render_last_stats", "rt_bvh_build", "rt_bvh_build_seeded", "rt_scene_lights",
     "rt_batch_create", "rt_batch_set_config", "rt_batch_get_config", "rt_batch_launch", "rt_batch_autotune",
     "rt_batch_destroy",
     "rt_host_alloc", "rt_host_free", "rt_server_create", "rt_server_destroy", "rt_server_scene_add", "rt_server_scene_remove", "rt_server_scene_list",
@@ -186,6 +195,8 @@ def lib():
         "rt_render": (i32, [P, P, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), P, P, u32, u32, P]),
         "rt_render_frames": (i32, [P, P, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), u32, P, P, u32, u32, P]),
         "rt_render_last_rays": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+        "rt_render_set_timing": (i32, [P, i32]),
+        "rt_render_last_stats": (i32, [P, ctypes.POINTER(RenderStats)]),
         "rt_batch_create": (i32, [P, ctypes.POINTER(BatchItem), u32, ctypes.POINTER(BatchConfig), ctypes.POINTER(P)]),
         "rt_batch_set_config": (i32, [P, P, ctypes.POINTER(BatchConfig)]),
         "rt_batch_get_config": (i32, [P, ctypes.POINTER(BatchConfig)]),
@@ -478,6 +489,14 @@ class Context:
         p, s = ctypes.c_uint64(), ctypes.c_uint64()
         self.check(lib().rt_render_last_rays(self._h, ctypes.byref(p), ctypes.byref(s)), "rt_render_last_rays")
         return p.value, s.value
+
+    def set_render_timing(self, enable: bool):
+        self.check(lib().rt_render_set_timing(self._h, 1 if enable else 0), "rt_render_set_timing")
+
+    def last_render_stats(self) -> RenderStats:
+        st = RenderStats()
+        self.check(lib().rt_render_last_stats(self._h, ctypes.byref(st)), "rt_render_last_stats")
+        return st
 
 
 class Batch:
