@@ -278,6 +278,42 @@ typedef struct rt_render_stats {
 int rt_render_set_timing(rt_ctx* ctx, int enable);
 int rt_render_last_stats(rt_ctx* ctx, rt_render_stats* out);
 
+/* -------------------------------------------------- frame loop and display */
+/* The reference's main loop around the compute pass (src/main.cpp:166-183
+ * updateScene, :228-261 the dispatch and the target->accumulation copy):
+ * rt_frame_loop_step resets currentSample to 0 when `moved`, fills the UBO
+ * (camera, time, currentSample, the scene's counts), renders into the
+ * loop's target reading its accumulation image, copies target to
+ * accumulation and increments currentSample.  The loop owns three device
+ * RGBA8 images of w*h*4 bytes: target, accumulation and display. */
+typedef struct rt_frame_loop rt_frame_loop;
+int rt_frame_loop_create(rt_ctx* ctx, const rt_scene* scene, const rt_render_params* params, uint32_t w,
+                         uint32_t h, rt_frame_loop** out);
+int rt_frame_loop_destroy(rt_frame_loop* loop);
+int rt_frame_loop_step(rt_frame_loop* loop, const float cam_pos[3], float time, int moved, void* stream);
+/* The display pass of post-process-shader.frag:62-68 applied to the target:
+ * display = pow(target, 1/2.2) per colour channel, alpha 255. */
+int rt_frame_loop_display(rt_frame_loop* loop, void* stream);
+/* Device pointers of the loop's images and the next frame's currentSample. */
+int rt_frame_loop_images(const rt_frame_loop* loop, uint8_t** target_dev, uint8_t** accum_dev,
+                         uint8_t** display_dev, uint32_t* current_sample);
+/* Copies image `which` (0 target, 1 accumulation, 2 display) to host memory
+ * of w*h*4 bytes after the work queued on `stream`; returns when done. */
+int rt_frame_loop_read(const rt_frame_loop* loop, int which, uint8_t* host, void* stream);
+/* The display pass on any RGBA8 image (device memory, 4-byte aligned):
+ * out = rne(pow(fl(c/255), fl(1/2.2)) * 255) per colour channel with pow
+ * correctly rounded to binary32, alpha 255.  The shader samples the target
+ * through a linear filter at texel centres, which returns the texel. */
+int rt_postprocess_gamma(rt_ctx* ctx, const uint8_t* src_dev, uint8_t* dst_dev, uint32_t w, uint32_t h,
+                         void* stream);
+/* The 256-entry channel map rt_postprocess_gamma applies. */
+int rt_gamma_table(uint8_t out[256]);
+/* Writes a host RGBA8 image (rows of w*4 bytes) to `path`: ".png" (RGBA,
+ * stored deflate blocks) or ".ppm" (binary P6, alpha dropped); other
+ * extensions give RT_ERR_UNSUPPORTED.  Not in the reference, which
+ * presents through the swapchain. */
+int rt_image_write(const char* path, const uint8_t* rgba, uint32_t w, uint32_t h);
+
 /* ------------------------------------------------------------------ served */
 /* A request loop on its own thread: scenes stay resident under an HBM
  * budget (least recently used scenes without pending work are evicted to
@@ -346,6 +382,18 @@ int rt_bvh_build(const rt_triangle* tris, uint32_t n, rt_bvh_node* out);
  * (seed 1 == a fresh process), run on `threads` host threads. */
 int rt_bvh_build_seeded(const rt_triangle* tris, uint32_t n, uint32_t seed,
                         uint32_t threads, rt_bvh_node* out);
+/* GPU builder (no counterpart in the reference, whose builder runs on the
+ * host, Bvh.h:141-209): a linear BVH (Karras, HPG 2012) over the box
+ * centres of the device triangles -- 63-bit Morton codes over the centres'
+ * bounds, a stable radix sort, a radix tree whose equal codes are ordered
+ * by sorted position -- written to nodes_dev in the reference's node
+ * layout: 2n-1 nodes, root at node 0, internal node k at index k, the leaf
+ * of sorted position k at index n-1+k, leaf boxes the reference's
+ * triangle boxes (bounds widened by 0.0001, Bvh.h:96-97), parent boxes the
+ * union of their children's; codes use the centres of those boxes.  The tree differs
+ * from Bvh::createBvh's; every closest hit is the same.  Both pointers are
+ * device memory, 16-byte aligned; runs on `stream`; n <= 2^30. */
+int rt_bvh_build_device(rt_ctx* ctx, const rt_triangle* tris_dev, uint32_t n, rt_bvh_node* nodes_dev, void* stream);
 /* Light list rule of RtScene.h:87-96: for each triangle whose material is
  * a light, Light{i, 0.5*|v0 x v1|}.  Writes up to `cap` lights and returns
  * the count in *count. */

@@ -96,6 +96,10 @@ def lib():
         L.orc_render.argtypes = [P, P, P, u32, u32, u32, P, P, P, ctypes.c_int]
         L.orc_render.restype = ctypes.c_uint64
         L.orc_camera_rays.argtypes = [P, ctypes.c_float, u32, u32, P]
+        L.orc_morton.restype = ctypes.c_uint64
+        L.orc_morton.argtypes = [u32, u32, u32]
+        L.orc_lbvh.restype = ctypes.c_int
+        L.orc_lbvh.argtypes = [P, u32, P, P, P]
         _lib = L
     return _lib
 
@@ -254,3 +258,22 @@ def camera_rays(cam_pos, w, h, vfov=30.0, current_sample=0):
     out = np.zeros((h * w, 6), np.float32)
     lib().orc_camera_rays(ctypes.byref(ubo), vfov, w, h, out.ctypes.data)
     return out
+
+
+def morton(x: int, y: int, z: int) -> int:
+    """63-bit Morton code of three 21-bit cells (x in bits 3b+2)."""
+    return int(lib().orc_morton(x, y, z))
+
+
+def lbvh(triangles: np.ndarray):
+    """The linear BVH of the GPU builder: (nodes [2n-1], sorted codes [n], order [n])."""
+    tris = np.ascontiguousarray(triangles, dtype=TRI_DT)
+    n = len(tris)
+    if n == 0:
+        raise ValueError("lbvh: no triangles")
+    nodes = np.zeros(2 * n - 1, NODE_DT)
+    codes = np.zeros(n, np.uint64)
+    order = np.zeros(n, np.uint32)
+    if lib().orc_lbvh(_ptr(tris), n, _ptr(codes), _ptr(order), _ptr(nodes)) != 0:
+        raise MemoryError("lbvh: allocation failed")
+    return nodes, codes, order

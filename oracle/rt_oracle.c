@@ -739,3 +739,144 @@ void orc_camera_rays(const orc_ubo* ubo, float vfov, uint32_t w, uint32_t h, flo
             o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = d.x; o[4] = d.y; o[5] = d.z;
         }
 }
+
+/* ------------------------------------------------------ linear BVH builder
+ * Restatement of the library's GPU builder (rt_bvh_build_device), a linear
+ * BVH after Karras (HPG 2012).  The reference has no such builder (its
+ * Bvh::createBvh, src/ray-tracing/Bvh.h:141-209, runs on the host), so the
+ * restatement follows the builder's own published definition:
+ *   box     the reference's triangle box (Bvh.h:96-97): min(v0,v1,v2) -
+ *           0.0001 and max(v0,v1,v2) + 0.0001 per axis
+ *   centre  c = (lo + hi) * 0.5 per axis
+ *   scale   s = 2^21 / (hi - lo) over the centres' bounds (0 if hi == lo,
+ *           and 0 when that quotient overflows)
+ *   code    q = (c - lo) * s clamped to [0, 2^21 - 1] (NaN -> 0); bit b of
+ *           q_x, q_y, q_z goes to code bit 3b+2, 3b+1, 3b
+ *   order   (code, triangle index) ascending
+ * The tree is built top-down, not by the kernel's per-node search: the
+ * sorted range [f, l] splits after the last position whose key agrees with
+ * f's on the highest bit where f's and l's keys differ (keys compare as
+ * (code, sorted position)).  Its children are named "g" and "g+1" for the
+ * split g: a leaf (node n-1+k for position k) when the side holds one
+ * position, else internal node g (left) or g+1 (right).  Boxes are the
+ * exact unions of the triangles' boxes over each range. */
+typedef struct { uint64_t code; uint32_t idx; } o_key;
+
+static uint64_t morton_bits(uint32_t x, uint32_t y, uint32_t z) {
+    uint64_t c = 0;
+    for (int b = 0; b < 21; ++b) {
+        c |= (uint64_t)((x >> b) & 1u) << (3 * b + 2);
+        c |= (uint64_t)((y >> b) & 1u) << (3 * b + 1);
+        c |= (uint64_t)((z >> b) & 1u) << (3 * b);
+    }
+    return c;
+}
+
+uint64_t orc_morton(uint32_t x, uint32_t y, uint32_t z) { return morton_bits(x, y, z); }
+
+static int key_cmp(const void* a, const void* b) {
+    const o_key* p = (const o_key*)a;
+    const o_key* q = (const o_key*)b;
+    if (p->code != q->code) return p->code < q->code ? -1 : 1;
+    return p->idx < q->idx ? -1 : (p->idx > q->idx);
+}
+
+/* the reference's triangle box (src/ray-tracing/Bvh.h:96-97): bounds
+ * widened by eps = 0.0001 */
+static void tri_bounds(const o_tri* t, float lo[3], float hi[3]) {
+    const float eps = 0.0001f;
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = gmin(gmin(t->v0[a], t->v1[a]), t->v2[a]) - eps;
+        hi[a] = gmax(gmax(t->v0[a], t->v1[a]), t->v2[a]) + eps;
+    }
+}
+
+static uint32_t quantise(float c, float lo, float s) {
+    float q = (c - lo) * s;
+    if (q >= 2097151.0f) return 2097151u;
+    return q > 0.0f ? (uint32_t)q : 0u;
+}
+
+static void write_node(o_node* nd, const float lo[3], const float hi[3], int32_t left, int32_t right, int32_t obj) {
+    memset(nd, 0, sizeof *nd);
+    for (int a = 0; a < 3; ++a) { nd->mn[a] = lo[a]; nd->mx[a] = hi[a]; }
+    nd->left = left; nd->right = right; nd->obj = obj;
+}
+
+/* Sorted keys and triangle order (n entries each) and 2n-1 nodes. */
+int orc_lbvh(const void* tris_v, uint32_t n, uint64_t* codes_out, uint32_t* order_out, void* nodes_v) {
+    const o_tri* tris = (const o_tri*)tris_v;
+    o_node* nodes = (o_node*)nodes_v;
+    if (n == 0) return 1;
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < n; ++i) {
+        float lo[3], hi[3];
+        tri_bounds(&tris[i], lo, hi);
+        for (int a = 0; a < 3; ++a) {
+            float c = (lo[a] + hi[a]) * 0.5f;
+            clo[a] = gmin(clo[a], c);
+            chi[a] = gmax(chi[a], c);
+        }
+    }
+    float s[3];
+    for (int a = 0; a < 3; ++a) {
+        float ext = chi[a] - clo[a];
+        float sv = ext > 0.0f ? 2097152.0f / ext : 0.0f;
+        s[a] = sv <= 3.0e38f ? sv : 0.0f;
+    }
+    o_key* keys = (o_key*)malloc(sizeof(o_key) * n);
+    if (!keys) return 2;
+    for (uint32_t i = 0; i < n; ++i) {
+        float lo[3], hi[3];
+        uint32_t q[3];
+        tri_bounds(&tris[i], lo, hi);
+        for (int a = 0; a < 3; ++a) q[a] = quantise((lo[a] + hi[a]) * 0.5f, clo[a], s[a]);
+        keys[i].code = morton_bits(q[0], q[1], q[2]);
+        keys[i].idx = i;
+    }
+    qsort(keys, n, sizeof(o_key), key_cmp);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (codes_out) codes_out[k] = keys[k].code;
+        if (order_out) order_out[k] = keys[k].idx;
+        float lo[3], hi[3];
+        tri_bounds(&tris[keys[k].idx], lo, hi);
+        write_node(&nodes[n - 1 + k], lo, hi, -1, -1, (int32_t)keys[k].idx);
+    }
+    if (n > 1) {
+        /* explicit stack of (first, last, node) */
+        uint32_t* st = (uint32_t*)malloc(sizeof(uint32_t) * 3 * (size_t)n);
+        if (!st) { free(keys); return 2; }
+        size_t sp = 0;
+        st[sp++] = 0; st[sp++] = n - 1; st[sp++] = 0;
+        while (sp) {
+            uint32_t node = st[--sp], l = st[--sp], f = st[--sp];
+            uint32_t g;
+            if (keys[f].code != keys[l].code) {
+                int p = 63 - __builtin_clzll(keys[f].code ^ keys[l].code);
+                /* first position whose code has bit p set (f's is clear, l's set) */
+                uint32_t a = f, b = l;
+                while (b - a > 1) {
+                    uint32_t m = a + (b - a) / 2;
+                    if ((keys[m].code >> p) & 1u) b = m; else a = m;
+                }
+                g = b - 1;
+            } else {
+                int p = 31 - __builtin_clz(f ^ l);
+                g = ((l >> p) << p) - 1;    /* positions below l's bit p go left */
+            }
+            int32_t left = g == f ? (int32_t)(n - 1 + g) : (int32_t)g;
+            int32_t right = g + 1 == l ? (int32_t)(n - 1 + g + 1) : (int32_t)(g + 1);
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t k = f; k <= l; ++k) {
+                const o_node* lf = &nodes[n - 1 + k];
+                for (int a = 0; a < 3; ++a) { lo[a] = gmin(lo[a], lf->mn[a]); hi[a] = gmax(hi[a], lf->mx[a]); }
+            }
+            write_node(&nodes[node], lo, hi, left, right, -1);
+            if (g != f) { st[sp++] = f; st[sp++] = g; st[sp++] = g; }
+            if (g + 1 != l) { st[sp++] = g + 1; st[sp++] = l; st[sp++] = g + 1; }
+        }
+        free(st);
+    }
+    free(keys);
+    return 0;
+}

@@ -30,6 +30,9 @@ SOURCES = [
     "rt_trace.hip",
     "rt_render.hip",
     "rt_batch.hip",
+    "rt_build.hip",
+    "rt_frame.hip",
+    "rt_image.cpp",
     "rt_server.cpp",
 ]
 

@@ -82,7 +82,7 @@ extern "C" int rt_destroy(rt_ctx* ctx) {
     if (!ctx) return RT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < rt_ctx::kWorkspaces; ++i)
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_cursors) (void)hipFree(ctx->d_cursors);

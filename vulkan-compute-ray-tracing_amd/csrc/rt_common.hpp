@@ -161,9 +161,11 @@ struct rt_ctx {
     std::string err;
     int num_cus = 0;
     // grow-only device workspaces: [0] render path state, [1] reference
-    // traversal stacks, [2] statistics scratch, [3] traversal stack spill
-    void* ws[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t ws_bytes[4] = {0, 0, 0, 0};
+    // traversal stacks, [2] statistics scratch, [3] traversal stack spill,
+    // [4] BVH builder scratch
+    static constexpr int kWorkspaces = 5;
+    void* ws[kWorkspaces] = {};
+    size_t ws_bytes[kWorkspaces] = {};
     uint32_t* d_counters = nullptr;   // small zero-filled counter block
     uint32_t* d_cursors = nullptr;    // ring of persistent-kernel ray cursors
     uint32_t cursor_next = 0;
@@ -182,7 +184,7 @@ namespace rtamd {
 
 int set_error(rt_ctx* ctx, int code, const std::string& msg);
 int hip_error(rt_ctx* ctx, hipError_t e, const char* what);
-enum { kWsRender = 0, kWsStack = 1, kWsStats = 2, kWsSpill = 3 };
+enum { kWsRender = 0, kWsStack = 1, kWsStats = 2, kWsSpill = 3, kWsBuild = 4 };
 constexpr uint32_t kCursorSlots = 1024;
 int ensure_workspace(rt_ctx* ctx, int slot, size_t bytes);
 

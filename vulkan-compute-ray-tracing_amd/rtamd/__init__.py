@@ -159,7 +159,9 @@ EXPORTS = [
     "rt_create", "rt_destroy", "rt_last_error", "rt_status_string", "rt_scene_upload", "rt_scene_info_get",
     "rt_scene_check", "rt_tuning_set", "rt_tuning_get",
     "rt_scene_release", "rt_closest_hit", "rt_closest_hit_stats", "rt_render", "rt_render_frames",
-    "rt_render_last_rays", "rt_render_set_timing", "rt_render_last_stats", "rt_bvh_build", "rt_bvh_build_seeded", "rt_scene_lights",
+    "rt_render_last_rays", "rt_render_set_timing", "rt_render_last_stats", "rt_bvh_build", "rt_bvh_build_seeded", "rt_bvh_build_device", "rt_scene_lights",
+    "rt_frame_loop_create", "rt_frame_loop_destroy", "rt_frame_loop_step", "rt_frame_loop_display",
+    "rt_frame_loop_images", "rt_frame_loop_read", "rt_postprocess_gamma", "rt_gamma_table", "rt_image_write",
     "rt_batch_create", "rt_batch_set_config", "rt_batch_get_config", "rt_batch_launch", "rt_batch_autotune",
     "rt_batch_destroy",
     "rt_host_alloc", "rt_host_free", "rt_server_create", "rt_server_destroy", "rt_server_scene_add", "rt_server_scene_remove", "rt_server_scene_list",
@@ -220,6 +222,16 @@ def lib():
         "rtw_serve_poisson": (i32, [P, ctypes.POINTER(ServeParams), ctypes.POINTER(ServeResult)]),
         "rt_bvh_build": (i32, [P, u32, P]),
         "rt_bvh_build_seeded": (i32, [P, u32, u32, u32, P]),
+        "rt_bvh_build_device": (i32, [P, P, u32, P, P]),
+        "rt_frame_loop_create": (i32, [P, P, ctypes.POINTER(RenderParams), u32, u32, ctypes.POINTER(P)]),
+        "rt_frame_loop_destroy": (i32, [P]),
+        "rt_frame_loop_step": (i32, [P, ctypes.POINTER(ctypes.c_float), ctypes.c_float, i32, P]),
+        "rt_frame_loop_display": (i32, [P, P]),
+        "rt_frame_loop_images": (i32, [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(u32)]),
+        "rt_frame_loop_read": (i32, [P, i32, P, P]),
+        "rt_postprocess_gamma": (i32, [P, P, P, u32, u32, P]),
+        "rt_gamma_table": (i32, [P]),
+        "rt_image_write": (i32, [ctypes.c_char_p, P, u32, u32]),
         "rt_scene_lights": (i32, [P, u32, P, u32, P, u32, ctypes.POINTER(u32)]),
         "rtw_scene_soup": (i32, [u32, u64, u32, ctypes.POINTER(WScene)]),
         "rtw_scene_icosphere": (i32, [u32, u64, ctypes.c_float, ctypes.c_float, u32, ctypes.POINTER(WScene)]),
@@ -490,6 +502,33 @@ class Context:
         self.check(lib().rt_render_last_rays(self._h, ctypes.byref(p), ctypes.byref(s)), "rt_render_last_rays")
         return p.value, s.value
 
+    def build_bvh_device(self, tris_dev, n: int, nodes_dev, stream=None):
+        """rt_bvh_build_device on device buffers (n 48-byte triangles in, 2n-1 48-byte nodes out)."""
+        self.check(lib().rt_bvh_build_device(self._h, _dev_ptr(tris_dev), n, _dev_ptr(nodes_dev), _stream_ptr(stream)),
+                   "rt_bvh_build_device")
+
+    def build_bvh(self, triangles) -> np.ndarray:
+        """The GPU linear BVH of host triangle records, returned as host node records."""
+        import torch
+        tris = np.ascontiguousarray(triangles, TRI_DT)
+        n = len(tris)
+        dev = f"cuda:{self.device}"
+        td = torch.from_numpy(tris.view(np.uint8).reshape(-1).copy()).to(dev)
+        nd = torch.empty(max(1, 2 * n - 1) * NODE_DT.itemsize, dtype=torch.uint8, device=dev)
+        self.build_bvh_device(td, n, nd)
+        torch.cuda.synchronize(dev)
+        return np.frombuffer(bytearray(nd.cpu().numpy().tobytes()), NODE_DT)
+
+    def postprocess_gamma(self, src, dst=None, stream=None):
+        """The display pass (rt_postprocess_gamma) of an [h, w, 4] uint8 device image."""
+        import torch
+        h, w = src.shape[0], src.shape[1]
+        if dst is None:
+            dst = torch.empty_like(src)
+        self.check(lib().rt_postprocess_gamma(self._h, _dev_ptr(src), _dev_ptr(dst), w, h, _stream_ptr(stream)),
+                   "rt_postprocess_gamma")
+        return dst
+
     def set_render_timing(self, enable: bool):
         self.check(lib().rt_render_set_timing(self._h, 1 if enable else 0), "rt_render_set_timing")
 
@@ -538,6 +577,73 @@ class Batch:
         if self.handle:
             self.ctx.check(lib().rt_batch_destroy(self.ctx.handle, self.handle), "rt_batch_destroy")
             self.handle = None
+
+
+class FrameLoop:
+    """An rt_frame_loop: the reference's per-frame sequence with its own
+    target, accumulation and display images on the device."""
+
+    def __init__(self, ctx: "Context", scene: "Scene", w: int, h: int, vfov=30.0, bounces=2, max_stack=0,
+                 spheres=0, traversal=TRAVERSE_AUTO):
+        self.ctx, self.scene, self.w, self.h = ctx, scene, w, h
+        prm = RenderParams(vfov, bounces, max_stack, spheres, traversal)
+        hnd = ctypes.c_void_p()
+        ctx.check(lib().rt_frame_loop_create(ctx.handle, scene.handle, ctypes.byref(prm), w, h, ctypes.byref(hnd)),
+                  "rt_frame_loop_create")
+        self.handle = hnd
+
+    def step(self, cam_pos, time=0.0, moved=False, stream=None):
+        cam = (ctypes.c_float * 3)(*[float(v) for v in cam_pos])
+        self.ctx.check(lib().rt_frame_loop_step(self.handle, cam, float(time), 1 if moved else 0, _stream_ptr(stream)),
+                       "rt_frame_loop_step")
+
+    def display(self, stream=None):
+        self.ctx.check(lib().rt_frame_loop_display(self.handle, _stream_ptr(stream)), "rt_frame_loop_display")
+
+    def _images(self):
+        t, a, d = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        cs = ctypes.c_uint32()
+        _check(lib().rt_frame_loop_images(self.handle, ctypes.byref(t), ctypes.byref(a), ctypes.byref(d),
+                                          ctypes.byref(cs)), what="rt_frame_loop_images")
+        return t.value, a.value, d.value, cs.value
+
+    @property
+    def current_sample(self) -> int:
+        return self._images()[3]
+
+    def image(self, which: str = "target", stream=None) -> np.ndarray:
+        """Copies one of the loop's images ("target", "accum", "display") to the host."""
+        out = np.empty((self.h, self.w, 4), np.uint8)
+        k = {"target": 0, "accum": 1, "display": 2}[which]
+        self.ctx.check(lib().rt_frame_loop_read(self.handle, k, out.ctypes.data, _stream_ptr(stream)),
+                       "rt_frame_loop_read")
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().rt_frame_loop_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gamma_table() -> np.ndarray:
+    out = np.zeros(256, np.uint8)
+    _check(lib().rt_gamma_table(out.ctypes.data), what="rt_gamma_table")
+    return out
+
+
+def image_write(path: str, rgba: np.ndarray):
+    """rt_image_write of an [h, w, 4] uint8 host image (.png or .ppm)."""
+    img = np.ascontiguousarray(rgba, np.uint8)
+    h, w, c = img.shape
+    if c != 4:
+        raise ValueError("image_write: expects [h, w, 4] RGBA8")
+    _check(lib().rt_image_write(path.encode(), img.ctypes.data, w, h), what=f"rt_image_write({path})")
 
 
 class Server:
