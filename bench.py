@@ -15,6 +15,8 @@ CPU baseline (the
 oracle's restatement of hit_bvh in the shader's visiting order, on a bounded
 sample of the same rays), and side measurements:
   render   the 1920x1080 x 4 spp, 8-bounce render of the same scene;
+  device_build  the GPU linear-BVH builder on the same triangles, the host
+           builder's time beside it, and the query on the device-built tree;
   batched  64 clustered-soup scenes (triangles log-uniform in [1k, 1M], seeds
            100+i) with ray batches log-uniform in [4k, 2M] (seeds 200+i):
            64 plain launches against one autotuned persistent launch
@@ -193,6 +195,39 @@ def cpu_baseline(hs, rays_np, seconds: float, threads: int):
                           "tri_tests_per_ray": round(float(tt.mean()), 3)}}
 
 
+def bench_device_build(ctx, hs, rays, threads: int, layout: int, node_bytes: int):
+    """GPU linear-BVH build of the query scene, the host builder beside it,
+    and the query on the device-built tree."""
+    import numpy as np
+    import torch
+    import rtamd
+    n = len(hs.triangles)
+    td = torch.from_numpy(np.ascontiguousarray(hs.triangles).view(np.uint8).reshape(-1).copy()).cuda()
+    nd = torch.empty((2 * n - 1) * 48, dtype=torch.uint8, device="cuda")
+    ms = median_ms(lambda: ctx.build_bvh_device(td, n, nd), 5)
+    t0 = time.perf_counter()
+    rtamd.bvh_build(hs.triangles, seed=1, threads=threads)
+    host_s = time.perf_counter() - t0
+    nodes = np.frombuffer(bytearray(nd.cpu().numpy().tobytes()), rtamd.NODE_DT)
+    dev = ctx.upload(rtamd.HostScene(hs.triangles, hs.materials, nodes, hs.lights, hs.spheres))
+    _, nv, tt = ctx.closest_hit_stats(dev, rays)
+    v_node = float(nv.to(torch.float64).mean().item())
+    v_tri = float(tt.to(torch.float64).mean().item())
+    hits = torch.empty((rays.shape[0], 4), dtype=torch.float32, device="cuda")
+    tq = median_ms(lambda: ctx.closest_hit(dev, rays, hits), 5)
+    bpr = v_node * node_bytes + v_tri * 48 + 48
+    out = {"workload": f"linear BVH (63-bit Morton codes, radix tree) of the {n}-triangle query scene, then the "
+                       "same incoherent query on that tree",
+           "build_ms": round(ms, 3), "host_builder_s": round(host_s, 3), "host_threads": threads,
+           "fast_path": int(dev.info.fast_path), "tree_height": int(dev.info.tree_height),
+           "query_ms": round(tq, 4), "mrays_per_s": round(rays.shape[0] / tq / 1e3, 1),
+           "node_records_per_ray": round(v_node, 2), "tri_records_per_ray": round(v_tri, 3),
+           "achieved_gbs": round(bpr * rays.shape[0] / (tq / 1e3) / 1e9, 1)}
+    log(f"device build: {ms:.2f} ms (host {host_s:.2f} s), query on it {out['mrays_per_s']} Mrays/s")
+    del td, nd, dev, hits
+    return out
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -367,6 +402,7 @@ def main():
                          "secondary_trace_mrays_per_s": round(sec / st.trace_secondary_ms / 1e3, 1)}
         log(f"render: {rms:.1f} ms, secondary traversal {out['render']['secondary_trace_mrays_per_s']} Mrays/s")
 
+    out["device_build"] = bench_device_build(ctx, hs, rays, threads, layout, node_bytes)
     out.update(bench_small_and_medium(ctx, threads))
 
     if not args.no_batched:

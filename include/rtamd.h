@@ -372,6 +372,20 @@ int rt_server_stats_get(rt_server* server, rt_server_stats* stats);
 int rt_server_stats_reset(rt_server* server);
 const char* rt_server_last_error(const rt_server* server);
 
+/* ---------------------------------------------------------------- OBJ input */
+/* Triangles of one Wavefront OBJ model, as GpuModel::getTriangles makes
+ * them (src/ray-tracing/RtScene.h:13-30 over Mesh::Mesh, src/scene/mesh.cpp
+ * :96-139): triangle k is the positions of face k's corners in file order
+ * (the reference's vertex de-duplication changes only its index buffer),
+ * each with `material_index`.  Faces with k > 3 corners become the fan
+ * (0, i, i+1); corner indices are 1-based or negative (relative); numbers
+ * are parsed as double and rounded to float.  *out is malloc'ed (release
+ * with rt_free); RT_ERR_NOT_FOUND for a missing file, RT_ERR_INVALID for a
+ * malformed one (rt_obj_last_error gives the line, per thread). */
+int rt_obj_load(const char* path, uint32_t material_index, rt_triangle** out, uint32_t* count);
+const char* rt_obj_last_error(void);
+void rt_free(void* p);
+
 /* --------------------------------------------------------------- host BVH */
 /* Bvh::createBvh (src/ray-tracing/Bvh.h:141-209): 2n-1 nodes into out[].
  * Consumes the C library's rand() exactly as the reference does (one call

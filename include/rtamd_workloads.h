@@ -45,6 +45,13 @@ int rtw_scene_icosphere(uint32_t level, uint64_t seed, float amp, float freq, ui
  * orientation; materials drawn 60/25/10/5 % lambertian/metal/glass/light. */
 int rtw_scene_clusters(uint32_t n, uint64_t seed, uint32_t threads, rtw_scene* out);
 
+/* The reference's own scene (src/ray-tracing/RtScene.h:44-101) from the OBJ
+ * files in `dir` (the reference's resources/models/doge_scene): its six
+ * materials, the eight models concatenated in its order with its material
+ * indices, its light rule, the glass sphere (0.6, 1, -1, r 0.6) and the
+ * reference builder's BVH (rt_bvh_build_seeded, seed 1). */
+int rtw_scene_doge(const char* dir, uint32_t threads, rtw_scene* out);
+
 void rtw_scene_free(rtw_scene* s);
 
 /* Camera rays of the shader's model (origin camPos.zxy*(-1,1,1), vfov, unit

@@ -33,6 +33,7 @@ SOURCES = [
     "rt_build.hip",
     "rt_frame.hip",
     "rt_image.cpp",
+    "rt_obj.cpp",
     "rt_server.cpp",
 ]
 

@@ -183,6 +183,8 @@ struct rt_ctx {
 namespace rtamd {
 
 int set_error(rt_ctx* ctx, int code, const std::string& msg);
+// Triangles of one OBJ file (rt_obj.cpp); err receives the reason on failure.
+int obj_triangles(const char* path, uint32_t material, std::vector<rt_triangle>& tris, std::string& err);
 int hip_error(rt_ctx* ctx, hipError_t e, const char* what);
 enum { kWsRender = 0, kWsStack = 1, kWsStats = 2, kWsSpill = 3, kWsBuild = 4 };
 constexpr uint32_t kCursorSlots = 1024;

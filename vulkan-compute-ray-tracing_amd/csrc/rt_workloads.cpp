@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -175,6 +176,39 @@ extern "C" int rtw_scene_soup(uint32_t n, uint64_t seed, uint32_t threads, rtw_s
         std::memset(spheres.data(), 0, sizeof(rt_sphere) * 2);
         spheres[0].s[0] = 0.6f; spheres[0].s[3] = 0.3f; spheres[0].materialIndex = 0;
         spheres[1].s[0] = -0.6f; spheres[1].s[3] = 0.3f; spheres[1].materialIndex = 0;
+        return finish(tris, mats, spheres, threads, out);
+    } catch (const std::bad_alloc&) {
+        return RT_ERR_NOMEM;
+    }
+}
+
+extern "C" int rtw_scene_doge(const char* dir, uint32_t threads, rtw_scene* out) {
+    if (!dir || !out) return RT_ERR_INVALID;
+    try {
+        // materials of RtScene.h:49-62, in that order
+        std::vector<rt_material> mats = {mat(RT_MAT_LAMBERTIAN, 0.3f, 0.3f, 0.3f), mat(RT_MAT_LAMBERTIAN, 0.9f, 0.1f, 0.1f),
+                                         mat(RT_MAT_LAMBERTIAN, 0.1f, 0.9f, 0.1f), mat(RT_MAT_LIGHT, 2.0f, 2.0f, 2.0f),
+                                         mat(RT_MAT_METAL, 1.0f, 1.0f, 1.0f), mat(RT_MAT_GLASS, 1.0f, 1.0f, 1.0f)};
+        // models and their materials, concatenated in the order of RtScene.h:81-88
+        struct Part {
+            const char* file;
+            uint32_t material;
+        };
+        const Part parts[] = {{"buff-doge.obj", 0}, {"cheems.obj", 0}, {"right.obj", 1}, {"left.obj", 2},
+                              {"back.obj", 0},      {"ceil.obj", 0},   {"floor.obj", 0}, {"light.obj", 3}};
+        std::vector<rt_triangle> tris;
+        for (const Part& pt : parts) {
+            std::string err;
+            const std::string path = std::string(dir) + "/" + pt.file;
+            int rc = rtamd::obj_triangles(path.c_str(), pt.material, tris, err);
+            if (rc) return rc;
+        }
+        if (tris.empty()) return RT_ERR_INVALID;
+        // the glass sphere of RtScene.h:97: centre (0.6, 1, -1), radius 0.6, material 5
+        std::vector<rt_sphere> spheres(1);
+        std::memset(spheres.data(), 0, sizeof(rt_sphere));
+        spheres[0].s[0] = 0.6f; spheres[0].s[1] = 1.0f; spheres[0].s[2] = -1.0f; spheres[0].s[3] = 0.6f;
+        spheres[0].materialIndex = 5;
         return finish(tris, mats, spheres, threads, out);
     } catch (const std::bad_alloc&) {
         return RT_ERR_NOMEM;

@@ -167,7 +167,8 @@ EXPORTS = [
     "rt_host_alloc", "rt_host_free", "rt_server_create", "rt_server_destroy", "rt_server_scene_add", "rt_server_scene_remove", "rt_server_scene_list",
     "rt_server_submit_query", "rt_server_submit_render", "rt_server_wait", "rt_server_drain", "rt_server_stats_get",
     "rt_server_stats_reset", "rt_server_last_error", "rtw_serve_poisson",
-    "rtw_scene_soup", "rtw_scene_icosphere", "rtw_scene_clusters", "rtw_scene_free", "rtw_rays_camera_tiles",
+    "rtw_scene_soup", "rtw_scene_icosphere", "rtw_scene_clusters", "rtw_scene_doge", "rtw_scene_free",
+    "rt_obj_load", "rt_obj_last_error", "rt_free", "rtw_rays_camera_tiles",
     "rtw_rays_shell", "rtw_rays_box", "rtw_log_uniform",
 ]
 
@@ -236,6 +237,10 @@ def lib():
         "rtw_scene_soup": (i32, [u32, u64, u32, ctypes.POINTER(WScene)]),
         "rtw_scene_icosphere": (i32, [u32, u64, ctypes.c_float, ctypes.c_float, u32, ctypes.POINTER(WScene)]),
         "rtw_scene_clusters": (i32, [u32, u64, u32, ctypes.POINTER(WScene)]),
+        "rtw_scene_doge": (i32, [ctypes.c_char_p, u32, ctypes.POINTER(WScene)]),
+        "rt_obj_load": (i32, [ctypes.c_char_p, u32, ctypes.POINTER(P), ctypes.POINTER(u32)]),
+        "rt_obj_last_error": (ctypes.c_char_p, []),
+        "rt_free": (None, [P]),
         "rtw_scene_free": (None, [ctypes.POINTER(WScene)]),
         "rtw_rays_camera_tiles": (i32, [P, ctypes.c_float, u32, u32, u32, u32, u32, P]),
         "rtw_rays_shell": (i32, [u32, u64, ctypes.c_float, ctypes.c_float, P]),
@@ -322,6 +327,28 @@ def scene_clusters(n: int, seed: int, threads: int = 8) -> HostScene:
     ws = WScene()
     _check(lib().rtw_scene_clusters(n, seed, threads, ctypes.byref(ws)), what="rtw_scene_clusters")
     return _take(ws)
+
+
+def scene_doge(models_dir: str, threads: int = 8) -> HostScene:
+    """The reference's doge scene (RtScene.h:44-101) from its OBJ directory."""
+    ws = WScene()
+    rc = lib().rtw_scene_doge(models_dir.encode(), threads, ctypes.byref(ws))
+    if rc != RT_OK:
+        raise RtError(rc, f"rtw_scene_doge({models_dir}): {lib().rt_obj_last_error().decode() or lib().rt_status_string(rc).decode()}")
+    return _take(ws)
+
+
+def obj_load(path: str, material_index: int = 0) -> np.ndarray:
+    """Triangle records of one OBJ model (rt_obj_load)."""
+    p, n = ctypes.c_void_p(), ctypes.c_uint32()
+    rc = lib().rt_obj_load(path.encode(), material_index, ctypes.byref(p), ctypes.byref(n))
+    if rc != RT_OK:
+        raise RtError(rc, f"rt_obj_load({path}): {lib().rt_obj_last_error().decode() or lib().rt_status_string(rc).decode()}")
+    try:
+        raw = ctypes.string_at(p.value, n.value * TRI_DT.itemsize) if n.value else b""
+        return np.frombuffer(bytearray(raw), TRI_DT).copy()
+    finally:
+        lib().rt_free(p.value)
 
 
 def rays_camera_tiles(cam_pos, vfov, w, h, tile, first, count) -> np.ndarray:
