@@ -146,11 +146,25 @@ enum {
     RT_TRAVERSE_PACKED = 1,     /* force packed ordered traversal            */
     RT_TRAVERSE_REFERENCE = 2   /* hit_bvh's own visiting order and line test */
 };
+/* Sphere rules.  The shader's main path ignores spheres (hit_bvh only);
+ * its unused hit_scene (ray-trace-compute.comp:222-247) tests them after
+ * the triangles with closest_so_far reset to t_max (comp:236), so a sphere
+ * hit replaces a nearer triangle hit.  Both rules run as a pass after the
+ * triangle traversal; scenes with more than 16 spheres walk a sphere tree
+ * built at upload (the reference's open "include spheres in bvh") that
+ * returns exactly what the loop over all spheres returns: the smallest
+ * accepted root, the highest index between equal roots.  A sphere hit
+ * reports id = num_triangles + sphere index, u = backFaceInt, v = 0. */
+enum {
+    RT_SPHERES_OFF = 0,         /* triangles only (the shader's hit_bvh)       */
+    RT_SPHERES_NEAREST = 1,     /* nearest of triangles and spheres            */
+    RT_SPHERES_SCENE = 2        /* hit_scene's rule: a sphere hit overrides    */
+};
 typedef struct rt_query_params {
     uint32_t traversal;   /* RT_TRAVERSE_*                                       */
     uint32_t max_stack;   /* reference mode only: 0 = unlimited, 16 = shader's
                              MAX_STACK_DEPTH truncation (ray-trace-compute.comp:262) */
-    uint32_t spheres;     /* 1: also test spheres after the BVH (closest wins)   */
+    uint32_t spheres;     /* RT_SPHERES_*                                        */
     uint32_t reserved;
 } rt_query_params;
 
@@ -235,7 +249,7 @@ typedef struct rt_render_params {
     uint32_t bounces;     /* shader NUM_BOUNCES = 2 (ray-trace-compute.comp:313) */
     uint32_t max_stack;   /* 0 = unlimited; 16 = reference truncation (reference
                              traversal only)                                    */
-    uint32_t spheres;     /* 0 = reference path (hit_bvh, triangles only)       */
+    uint32_t spheres;     /* RT_SPHERES_*; 0 = the shader's path (hit_bvh)      */
     uint32_t traversal;   /* RT_TRAVERSE_*                                      */
     uint32_t reserved[3];
 } rt_render_params;

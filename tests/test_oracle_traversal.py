@@ -134,3 +134,18 @@ def test_bug_compatible_sphere_rule(rt):
     assert sph.sum() >= (correct["id"] >= ntri).sum()
     # the bug shows: a sphere wins although a triangle is nearer
     assert ((correct["id"] < ntri) & (correct["id"] >= 0) & sph).any()
+
+
+def test_equal_sphere_roots_go_to_the_later_sphere(rt):
+    """hit_sphere accepts root == tMax (ray-trace-compute.comp:192, `tMax <
+    root` rejects), so in the sphere loop a later sphere at the same t
+    replaces an earlier one."""
+    hs = rt.scene_soup(16, 3)
+    sp = np.zeros(3, O.SPHERE_DT)
+    sp["s"] = [[5.0, 5.0, 5.0, 0.5], [5.0, 5.0, 5.0, 0.5], [9.0, 9.0, 9.0, 0.1]]
+    sc = O.Scene(hs.triangles, hs.materials, hs.nodes, hs.lights, sp)
+    rays = np.zeros(1, O.RAY_DT)
+    rays["o"], rays["d"], rays["tmin"], rays["tmax"] = [5.0, 5.0, 2.0], [0.0, 0.0, 1.0], 0.001, 10000.0
+    for rule in (1, 2):
+        h = O.closest_hit(sc, rays, O.TRAV_BRUTE, spheres=rule)
+        assert h["id"][0] == len(hs.triangles) + 1 and h["t"][0] == np.float32(2.5)

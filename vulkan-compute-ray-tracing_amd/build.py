@@ -32,6 +32,7 @@ SOURCES = [
     "rt_batch.hip",
     "rt_build.hip",
     "rt_frame.hip",
+    "rt_sphere.hip",
     "rt_image.cpp",
     "rt_obj.cpp",
     "rt_server.cpp",

@@ -127,17 +127,22 @@ int query_common(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays, uint32_
     rt_query_params p{};
     if (params) p = *params;
     if (p.traversal > RT_TRAVERSE_REFERENCE) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: unknown traversal");
+    if (p.spheres > RT_SPHERES_SCENE) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: unknown sphere rule");
     RT_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)stream;
     TraceArgs a{scene, rays, n, nullptr, hits, p.spheres, nv, tt};
     bool packed = p.traversal == RT_TRAVERSE_PACKED ||
                   (p.traversal == RT_TRAVERSE_AUTO && scene->fast && p.max_stack == 0);
     if (nv || tt) packed = true;
+    int rc;
     if (packed) {
         if (p.max_stack) return set_error(ctx, RT_ERR_UNSUPPORTED, "rt_closest_hit: max_stack applies to reference traversal");
-        return launch_trace_packed(ctx, a, s);
+        rc = launch_trace_packed(ctx, a, s);
+    } else {
+        rc = launch_trace_reference(ctx, a, p.max_stack, s);
     }
-    return launch_trace_reference(ctx, a, p.max_stack, s);
+    if (rc) return rc;
+    return launch_sphere_pass(ctx, a, s);
 }
 
 }  // namespace
@@ -168,6 +173,7 @@ int render_common(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo, const r
     if (((uintptr_t)target & 3u) || ((uintptr_t)accum_in & 3u))
         return set_error(ctx, RT_ERR_INVALID, "render: images must be 4-byte aligned");
     if (params->traversal > RT_TRAVERSE_REFERENCE) return set_error(ctx, RT_ERR_INVALID, "render: unknown traversal");
+    if (params->spheres > RT_SPHERES_SCENE) return set_error(ctx, RT_ERR_INVALID, "render: unknown sphere rule");
     if (!(params->vfov_deg > 0.0f && params->vfov_deg < 180.0f))
         return set_error(ctx, RT_ERR_INVALID, "render: vfov must lie in (0, 180) degrees");
     RT_HIP(ctx, hipSetDevice(ctx->device));

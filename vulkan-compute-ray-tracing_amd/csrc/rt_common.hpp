@@ -133,6 +133,22 @@ void widen_nodes(PackedTree& out);
 
 }  // namespace rtamd
 
+namespace rtamd {
+// Node of a scene's sphere tree (built at upload when a scene holds more
+// than kSphereLinear spheres): its box, and two child node indices, or for
+// a leaf left = -1 - sphere index and right = -1.
+struct SphereNode {
+    float lo[3];
+    int32_t left;
+    float hi[3];
+    int32_t right;
+};
+static_assert(sizeof(SphereNode) == 32, "SphereNode is 32 bytes");
+constexpr uint32_t kSphereLinear = 16;     // up to this many: a plain loop
+constexpr uint32_t kSphereStack = 32;      // LDS stack entries per lane
+constexpr uint32_t kMaxTreeSpheres = 1u << 24;   // median split: depth <= 24
+}  // namespace rtamd
+
 struct rt_scene {
     rt_ctx* ctx = nullptr;
     rt_scene_info info{};
@@ -153,6 +169,7 @@ struct rt_scene {
     uint32_t packed_height = 0;
     uint32_t ref_height = 0;
     bool fast = false;
+    rtamd::SphereNode* d_snodes = nullptr;   // sphere tree, or null: plain loop
 };
 
 struct rt_ctx {
@@ -211,6 +228,9 @@ struct TraceArgs {
 int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
 int resolved_layout(const rt_ctx* ctx);   // RT_LAYOUT_* actually used
 int launch_trace_reference(rt_ctx* ctx, const TraceArgs& a, uint32_t max_stack, hipStream_t s);
+// Spheres after a traversal (rt_sphere.hip): no-op unless a.spheres is set
+// and the scene has spheres.
+int launch_sphere_pass(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
 int launch_fill_u32(rt_ctx* ctx, uint32_t* p, uint32_t value, uint32_t count, hipStream_t s);
 
 // Batched traversal from caller-filled device tables (rt_batch.hip): the

@@ -326,6 +326,8 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
         if (ev) RT_HIP(ctx, hipEventRecord(ev[1 + 2 * b], s));
         rc = use_packed ? launch_trace_packed(ctx, ta, s) : launch_trace_reference(ctx, ta, prm.max_stack, s);
         if (rc) return rc;
+        rc = launch_sphere_pass(ctx, ta, s);
+        if (rc) return rc;
         if (ev) RT_HIP(ctx, hipEventRecord(ev[2 + 2 * b], s));
         hipLaunchKernelGGL(k_shade, dim3(grid_for(P, cu_cap)), dim3(256), 0, s, rays[b & 1], pids[b & 1], n_dev, P,
                            hits, rays[(b + 1) & 1], pids[(b + 1) & 1], counters + b + 1, color, rng, sc->d_tris,

@@ -4,6 +4,7 @@
 // storage buffers (main.cpp:84-106, Buffer.h:83-102) and the descriptor
 // binding of ComputeMaterial (ComputeMaterial.cpp:15-26, Material.cpp:258-322).
 // Unlike Buffer.h:85, each array is bound with its whole length.
+#include <functional>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -310,11 +311,76 @@ int upload_array(rt_ctx* ctx, T** dst, const T* src, uint32_t n, uint64_t& bytes
     return RT_OK;
 }
 
+// Sphere tree: median split on the longest axis of the centres' bounds
+// (ties by sphere index), one sphere per leaf, boxes = the spheres' bounds
+// padded by 2^-10 (|centre| + radius) + 1e-6 so rounding in the sphere
+// test never puts an accepted root outside its box.  Depth <= ceil(log2 n).
+// Returns false (keep the plain loop) for non-finite spheres.
+bool build_sphere_tree(const rt_sphere* sp, uint32_t n, std::vector<SphereNode>& out) {
+    std::vector<float> lo(3 * (size_t)n), hi(3 * (size_t)n);
+    for (uint32_t j = 0; j < n; ++j) {
+        const float* s4 = sp[j].s;
+        if (!std::isfinite(s4[0]) || !std::isfinite(s4[1]) || !std::isfinite(s4[2]) || !std::isfinite(s4[3]))
+            return false;
+        const float r = std::fabs(s4[3]);
+        const float m = std::max(std::max(std::fabs(s4[0]), std::fabs(s4[1])), std::fabs(s4[2]));
+        const float pad = (m + r) * 0x1p-10f + 1e-6f;
+        for (int a = 0; a < 3; ++a) {
+            lo[3 * (size_t)j + a] = s4[a] - r - pad;
+            hi[3 * (size_t)j + a] = s4[a] + r + pad;
+        }
+    }
+    std::vector<uint32_t> idx(n);
+    for (uint32_t j = 0; j < n; ++j) idx[j] = j;
+    out.clear();
+    out.reserve(2 * (size_t)n);
+    // explicit recursion on [b, e) returning the node index
+    std::function<int32_t(uint32_t, uint32_t)> rec = [&](uint32_t b, uint32_t e) -> int32_t {
+        const int32_t me = (int32_t)out.size();
+        out.push_back(SphereNode{});
+        if (e - b == 1) {
+            const uint32_t j = idx[b];
+            SphereNode& nd = out[me];
+            for (int a = 0; a < 3; ++a) { nd.lo[a] = lo[3 * (size_t)j + a]; nd.hi[a] = hi[3 * (size_t)j + a]; }
+            nd.left = -1 - (int32_t)j;
+            nd.right = -1;
+            return me;
+        }
+        float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t k = b; k < e; ++k)
+            for (int a = 0; a < 3; ++a) {
+                const float c = sp[idx[k]].s[a];
+                cmin[a] = std::min(cmin[a], c);
+                cmax[a] = std::max(cmax[a], c);
+            }
+        int axis = 0;
+        for (int a = 1; a < 3; ++a)
+            if (cmax[a] - cmin[a] > cmax[axis] - cmin[axis]) axis = a;
+        const uint32_t mid = b + (e - b) / 2;
+        std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e, [&](uint32_t x, uint32_t y) {
+            const float cx = sp[x].s[axis], cy = sp[y].s[axis];
+            return cx < cy || (cx == cy && x < y);
+        });
+        const int32_t l = rec(b, mid);
+        const int32_t r = rec(mid, e);
+        SphereNode& nd = out[me];
+        for (int a = 0; a < 3; ++a) {
+            nd.lo[a] = std::min(out[l].lo[a], out[r].lo[a]);
+            nd.hi[a] = std::max(out[l].hi[a], out[r].hi[a]);
+        }
+        nd.left = l;
+        nd.right = r;
+        return me;
+    };
+    rec(0, n);
+    return true;
+}
+
 void free_scene(rt_scene* s) {
     if (!s) return;
     for (void* p : {(void*)s->d_tris, (void*)s->d_mats, (void*)s->d_nodes, (void*)s->d_lights,
                     (void*)s->d_spheres, (void*)s->d_pnodes, (void*)s->d_qnodes, (void*)s->d_w4nodes,
-                    (void*)s->d_ptris,
+                    (void*)s->d_ptris, (void*)s->d_snodes,
                     (void*)s->d_ptri_id})
         if (p) (void)hipFree(p);
     delete s;
@@ -355,6 +421,17 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene*
         if ((rc = upload_array(ctx, &s->d_nodes, desc->nodes, desc->num_nodes, bytes))) break;
         if ((rc = upload_array(ctx, &s->d_lights, desc->lights, desc->num_lights, bytes))) break;
         if ((rc = upload_array(ctx, &s->d_spheres, desc->spheres, desc->num_spheres, bytes))) break;
+        if (desc->num_spheres > kSphereLinear && desc->num_spheres <= kMaxTreeSpheres) {
+            std::vector<SphereNode> snodes;
+            bool built = false;
+            try {
+                built = build_sphere_tree(desc->spheres, desc->num_spheres, snodes);
+            } catch (const std::bad_alloc&) {
+                rc = set_error(ctx, RT_ERR_NOMEM, "rt_scene_upload: sphere tree");
+                break;
+            }
+            if (built && (rc = upload_array(ctx, &s->d_snodes, snodes.data(), (uint32_t)snodes.size(), bytes))) break;
+        }
         if (fast) {
             if ((rc = upload_array(ctx, &s->d_pnodes, packed.nodes.data(), (uint32_t)packed.nodes.size(), bytes))) break;
             if ((rc = upload_array(ctx, &s->d_qnodes, packed.qnodes.data(), (uint32_t)packed.qnodes.size(), bytes))) break;

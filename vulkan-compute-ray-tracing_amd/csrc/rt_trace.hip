@@ -70,7 +70,7 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace_reference(
     const rt_bvh_node* __restrict__ nodes, const rt_triangle* __restrict__ tris, const rt_ray* __restrict__ rays,
     uint32_t n_host, const uint32_t* __restrict__ n_dev, rt_hit* __restrict__ hits, int32_t* __restrict__ stack,
-    uint32_t cap, uint32_t max_stack, const rt_sphere* __restrict__ spheres, uint32_t nspheres, uint32_t ntri) {
+    uint32_t cap, uint32_t max_stack) {
     const uint32_t n = n_dev ? *n_dev : n_host;
     const uint32_t stride = gridDim.x * BLOCK;
     const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
@@ -101,18 +101,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_reference(
             stack[(size_t)sp * stride + g] = nd.rightNodeIndex;
             sp++;
         }
-        int32_t id = best;
-        float best_t = closest;
-        if (nspheres) {
-            float cl = best >= 0 ? closest : r.tmax;
-            for (uint32_t j = 0; j < nspheres; ++j) {
-                float root_t; int back;
-                if (sphere_test(spheres[j], r.o, r.d, r.tmin, cl, root_t, back)) {
-                    cl = root_t; best_t = root_t; id = (int32_t)(ntri + j); bu = (float)back; bv = 0.0f;
-                }
-            }
-        }
-        if (id >= 0) store_hit(hits, i, id, best_t, bu, bv);
+        if (best >= 0) store_hit(hits, i, best, closest, bu, bv);
         else store_hit(hits, i, -1, 0.0f, 0.0f, 0.0f);
     }
 }
@@ -160,9 +149,8 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
     // 1..64 idle lanes trigger a refetch (64: only when the whole wave is idle)
     uint32_t refill = ctx->tuning.refill ? ctx->tuning.refill : 32u;
     if (refill > 64u) refill = 64u;
-    TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->d_spheres, sc->frame,
-                  LAYOUT == kLayoutW4 ? sc->root4 : sc->root_ref, a.spheres ? sc->info.num_spheres : 0u,
-                  sc->info.num_triangles};
+    TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->frame,
+                  LAYOUT == kLayoutW4 ? sc->root4 : sc->root_ref};
     TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, ts, io, spill);
     RT_HIP(ctx, hipGetLastError());
@@ -231,8 +219,7 @@ int launch_trace_reference(rt_ctx* ctx, const TraceArgs& a, uint32_t max_stack, 
     int rc = ensure_workspace(ctx, kWsStack, bytes);
     if (rc) return rc;
     hipLaunchKernelGGL((k_trace_reference<kBlock>), dim3(blocks), dim3(kBlock), 0, s, sc->d_nodes, sc->d_tris, a.rays,
-                       a.n, a.n_dev, a.hits, (int32_t*)ctx->ws[kWsStack], cap, max_stack, sc->d_spheres,
-                       a.spheres ? sc->info.num_spheres : 0u, sc->info.num_triangles);
+                       a.n, a.n_dev, a.hits, (int32_t*)ctx->ws[kWsStack], cap, max_stack);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;
 }

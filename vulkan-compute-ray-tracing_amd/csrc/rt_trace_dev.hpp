@@ -103,11 +103,8 @@ struct TraceScene {
     const PNode4* w4nodes;    // kLayoutW4
     const PTri* tris;
     const int32_t* tri_id;
-    const rt_sphere* spheres;
     QFrame frame;
     int32_t root;
-    uint32_t nspheres;
-    uint32_t ntri;
 };
 struct TraceIo {
     const rt_ray* rays;
@@ -327,18 +324,8 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                 if (sp > 0) {
                     cur = pop();
                 } else {
-                    // ---- ray finished: spheres, then the result
-                    int32_t id = best >= 0 ? l_triid[best] : -1;
-                    if (!BATCHED && sc.nspheres) {
-                        float closest = best >= 0 ? best_t : r.tmax;
-                        for (uint32_t j = 0; j < sc.nspheres; ++j) {
-                            float root_t; int back;
-                            if (sphere_test(sc.spheres[j], r.o, r.d, r.tmin, closest, root_t, back)) {
-                                closest = root_t; best_t = root_t; id = (int32_t)(sc.ntri + j);
-                                bu = (float)back; bv = 0.0f;
-                            }
-                        }
-                    }
+                    // ---- ray finished: the result (spheres follow in their own pass)
+                    const int32_t id = best >= 0 ? l_triid[best] : -1;
                     if (id >= 0) store_hit(l_hits, ray_i, id, best_t, bu, bv);
                     else store_hit(l_hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
                     if (STATS) { io.node_visits[ray_i] = nv; io.tri_tests[ray_i] = nt; }

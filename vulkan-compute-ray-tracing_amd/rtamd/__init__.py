@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(PKG_DIR, "librtamd.so")
 
 RT_OK, RT_ERR_INVALID, RT_ERR_HIP, RT_ERR_NOMEM, RT_ERR_UNSUPPORTED, RT_ERR_BUSY, RT_ERR_CANCELLED, RT_ERR_NOT_FOUND = range(8)
 RT_MAT_LIGHT, RT_MAT_LAMBERTIAN, RT_MAT_METAL, RT_MAT_GLASS = range(4)
+SPHERES_OFF, SPHERES_NEAREST, SPHERES_SCENE = range(3)
 TRAVERSE_AUTO, TRAVERSE_PACKED, TRAVERSE_REFERENCE = range(3)
 LAYOUT_AUTO, LAYOUT_F32, LAYOUT_Q16, LAYOUT_W4 = range(4)
 
