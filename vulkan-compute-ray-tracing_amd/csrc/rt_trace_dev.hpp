@@ -27,6 +27,23 @@ __device__ __forceinline__ void store_hit(rt_hit* hits, uint32_t i, int32_t id, 
     reinterpret_cast<float4*>(hits)[i] = h;
 }
 
+// Streaming forms for the persistent traversal: each ray is read once and
+// each hit written once, so both carry the non-temporal hint and leave the
+// caches to the node and triangle records, which every ray re-reads.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ RayIn load_ray_stream(const rt_ray* rays, uint32_t i) {
+    const v4f* p = reinterpret_cast<const v4f*>(rays) + 2 * (size_t)i;
+    const v4f a = __builtin_nontemporal_load(p);
+    const v4f b = __builtin_nontemporal_load(p + 1);
+    return RayIn{f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, a.w, b.w};
+}
+
+__device__ __forceinline__ void store_hit_stream(rt_hit* hits, uint32_t i, int32_t id, float t, float u, float v) {
+    const v4f h = {__int_as_float(id), t, u, v};
+    __builtin_nontemporal_store(h, reinterpret_cast<v4f*>(hits) + i);
+}
+
 // Conservative slab test of one box against [tmin, tmax].  Rounding of the
 // reciprocal form is absorbed by a relative slack of 2^-16 (far larger than
 // the few-ulp error), so a box that holds an accepted hit is never culled.
@@ -219,10 +236,10 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                         root = e->root;
                         q = e->frame;
                         ray_i = bio.perm ? bio.perm[mine] : mine - e->base;
-                        r = load_ray(e->rays, ray_i);
+                        r = load_ray_stream(e->rays, ray_i);
                     } else {
                         ray_i = mine;
-                        r = load_ray(io.rays, mine);
+                        r = load_ray_stream(io.rays, mine);
                     }
                     inv = f3{safe_inv(r.d.x), safe_inv(r.d.y), safe_inv(r.d.z)};
                     bo = r.o;
@@ -326,8 +343,8 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                 } else {
                     // ---- ray finished: the result (spheres follow in their own pass)
                     const int32_t id = best >= 0 ? l_triid[best] : -1;
-                    if (id >= 0) store_hit(l_hits, ray_i, id, best_t, bu, bv);
-                    else store_hit(l_hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
+                    if (id >= 0) store_hit_stream(l_hits, ray_i, id, best_t, bu, bv);
+                    else store_hit_stream(l_hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
                     if (STATS) { io.node_visits[ray_i] = nv; io.tri_tests[ray_i] = nt; }
                     busy = false;
                 }
