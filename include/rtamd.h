@@ -183,8 +183,16 @@ int rt_closest_hit_stats(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays_
 /* ------------------------------------------------------------------ tuning */
 /* Launch parameters of the packed traversal (no counterpart in the
  * reference, whose only knob is the compile-time MAX_STACK_DEPTH).  Zero
- * fields keep the library default.  They never change results. */
+ * fields keep the library default.  They never change results, except
+ * `tree`, which can change which of two hits at equal t is reported. */
 enum { RT_LAYOUT_AUTO = 0, RT_LAYOUT_F32 = 1, RT_LAYOUT_Q16 = 2, RT_LAYOUT_W4 = 3 };
+/* Tree of the packed traversal, read by rt_scene_upload:
+ * RT_TREE_GIVEN   the caller's nodes (e.g. Bvh::createBvh's)
+ * RT_TREE_DEVICE  a linear BVH built on the device at upload
+ *                 (rt_bvh_build_device); the shader-order traversal keeps the
+ *                 caller's nodes, and a scene given without nodes uses the
+ *                 device tree for both */
+enum { RT_TREE_GIVEN = 0, RT_TREE_DEVICE = 1 };
 typedef struct rt_tuning {
     uint32_t node_layout;   /* RT_LAYOUT_*: binary 64-byte fp32 or 32-byte 16-bit
                                nodes, or 4-wide 64-byte 16-bit nodes            */
@@ -193,7 +201,8 @@ typedef struct rt_tuning {
     uint32_t stack_lds;     /* 4-wide layout: 0 = whole traversal stack in LDS;
                                8 or 16 = that many top entries in LDS, deeper ones
                                spill to HBM (more waves fit per CU)               */
-    uint32_t reserved[4];
+    uint32_t tree;          /* RT_TREE_*                                          */
+    uint32_t reserved[3];
 } rt_tuning;
 int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t);
 int rt_tuning_get(const rt_ctx* ctx, rt_tuning* t);

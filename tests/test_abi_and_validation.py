@@ -47,6 +47,16 @@ def test_record_layouts_match_reference(rt):
     assert ctypes.sizeof(rt.Ubo) == 32 and rt.Ubo.currentSample.offset == 16
 
 
+def test_parameter_structs_match_the_header(rt):
+    """ctypes mirrors of the C-ABI's parameter structs (include/rtamd.h)."""
+    import ctypes
+    sizes = {"Tuning": 32, "QueryParams": 16, "RenderParams": 32, "RenderStats": 40, "BatchItem": 32,
+             "BatchConfig": 32, "ServerConfig": 40, "SceneInfo": 48}
+    for name, size in sizes.items():
+        assert ctypes.sizeof(getattr(rt, name)) == size, name
+    assert rt.Tuning.tree.offset == 16 and rt.RenderStats.total_ms.offset == 24
+
+
 def test_oracle_layouts_agree(rt, orc):
     for a, b in [(rt.TRI_DT, orc.TRI_DT), (rt.MAT_DT, orc.MAT_DT), (rt.NODE_DT, orc.NODE_DT),
                  (rt.LIGHT_DT, orc.LIGHT_DT), (rt.SPHERE_DT, orc.SPHERE_DT), (rt.RAY_DT, orc.RAY_DT)]:

@@ -104,3 +104,49 @@ def test_device_build_rejects_bad_input(rt, gpu_ctx):
     nd = torch.zeros(48, dtype=torch.uint8, device="cuda")
     with pytest.raises(rt.RtError):
         gpu_ctx.build_bvh_device(t, 0, nd)
+
+
+def test_upload_with_device_tree(rt, gpu_ctx):
+    """RT_TREE_DEVICE: the packed traversal runs on the device-built tree,
+    exactly as if the caller had passed that tree; the shader-order
+    traversal keeps the caller's nodes."""
+    hs = rt.scene_clusters(200000, 13)
+    rays = rt.rays_to_device(rt.rays_box(1 << 18, 14, 4.0))
+    print("case: upload with RT_TREE_DEVICE, 200k triangles", flush=True)
+    explicit = gpu_ctx.upload(rt.HostScene(hs.triangles, hs.materials, gpu_ctx.build_bvh(hs.triangles), hs.lights))
+    given = gpu_ctx.upload(hs)
+    gpu_ctx.set_tuning(tree=rt.TREE_DEVICE)
+    try:
+        dev = gpu_ctx.upload(hs)
+    finally:
+        gpu_ctx.set_tuning()
+    assert dev.info.fast_path == 1 and dev.info.num_nodes == len(hs.nodes)
+    a = rt.hits_from_device(gpu_ctx.closest_hit(dev, rays))
+    b = rt.hits_from_device(gpu_ctx.closest_hit(explicit, rays))
+    assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    c = rt.hits_from_device(gpu_ctx.closest_hit(dev, rays, traversal=rt.TRAVERSE_REFERENCE))
+    d = rt.hits_from_device(gpu_ctx.closest_hit(given, rays, traversal=rt.TRAVERSE_REFERENCE))
+    assert np.array_equal(c.view(np.uint8), d.view(np.uint8))
+
+
+def test_upload_without_nodes(rt, gpu_ctx):
+    """A scene given without a BVH: RT_TREE_DEVICE builds one for both
+    traversals; RT_TREE_GIVEN rejects it."""
+    base = rt.scene_soup(1024, 1)
+    bare = rt.HostScene(base.triangles, base.materials, None, base.lights, base.spheres)
+    with pytest.raises(rt.RtError):
+        gpu_ctx.upload(bare)
+    print("case: upload without nodes, RT_TREE_DEVICE", flush=True)
+    gpu_ctx.set_tuning(tree=rt.TREE_DEVICE)
+    try:
+        sc = gpu_ctx.upload(bare)
+    finally:
+        gpu_ctx.set_tuning()
+    nodes = gpu_ctx.build_bvh(base.triangles)
+    assert sc.info.num_nodes == len(nodes) and sc.info.fast_path == 1
+    rays = np.concatenate([rt.rays_shell(4096, 15, 3.0, 0.5), rt.rays_box(4096, 16, 1.2)])
+    want = O.closest_hit(O.Scene(base.triangles, base.materials, nodes, base.lights, base.spheres), rays,
+                         O.TRAV_REFERENCE, threads=8)
+    got = rt.hits_from_device(gpu_ctx.closest_hit(sc, rt.rays_to_device(rays), traversal=rt.TRAVERSE_REFERENCE))
+    assert np.array_equal(got["id"], want["id"])
+    assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))

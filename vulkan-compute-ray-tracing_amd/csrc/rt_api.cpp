@@ -96,6 +96,7 @@ extern "C" int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t) {
     if (!ctx || !t) return RT_ERR_INVALID;
     if (t->node_layout > RT_LAYOUT_W4) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: unknown node layout");
     if (t->refill > 64) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: refill must be 0..64");
+    if (t->tree > RT_TREE_DEVICE) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: unknown tree");
     if (t->stack_lds != 0 && t->stack_lds != 8 && t->stack_lds != 16)
         return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: stack_lds must be 0, 8 or 16");
     ctx->tuning = *t;

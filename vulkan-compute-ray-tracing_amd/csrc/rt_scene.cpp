@@ -387,29 +387,74 @@ void free_scene(rt_scene* s) {
 }
 }  // namespace
 
+// The linear BVH of the scene's triangles, built on the device and copied
+// back (RT_TREE_DEVICE uploads).
+int device_tree(rt_ctx* ctx, const rt_triangle* tris, uint32_t n, std::vector<rt_bvh_node>& out) {
+    if (!tris || n == 0) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: scene has no triangles");
+    if (n > (1u << 30)) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: more than 2^30 triangles");
+    const size_t tb = sizeof(rt_triangle) * (size_t)n, nb = sizeof(rt_bvh_node) * (2 * (size_t)n - 1);
+    rt_triangle* dt = nullptr;
+    rt_bvh_node* dn = nullptr;
+    int rc = RT_OK;
+    hipError_t e = hipMalloc((void**)&dt, tb);
+    if (e == hipSuccess) e = hipMalloc((void**)&dn, nb);
+    if (e == hipSuccess) e = hipMemcpy(dt, tris, tb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_error(ctx, e, "rt_scene_upload: device tree");
+    if (!rc) rc = rt_bvh_build_device(ctx, dt, n, dn, nullptr);
+    if (!rc) {
+        try {
+            out.resize(2 * (size_t)n - 1);
+        } catch (const std::bad_alloc&) {
+            rc = set_error(ctx, RT_ERR_NOMEM, "rt_scene_upload: device tree");
+        }
+    }
+    if (!rc) {
+        e = hipMemcpy(out.data(), dn, nb, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_error(ctx, e, "rt_scene_upload: device tree");
+    }
+    if (dt) (void)hipFree(dt);
+    if (dn) (void)hipFree(dn);
+    return rc;
+}
+
 extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
     if (!ctx) return RT_ERR_INVALID;
     if (!desc || !out) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: null argument");
     *out = nullptr;
-    SceneCheck chk;
+    RT_HIP(ctx, hipSetDevice(ctx->device));
+    // `user` keeps the caller's tree (the shader-order traversal walks it),
+    // `packsrc` is the tree the packed traversal is built from
+    rt_scene_desc user = *desc, packsrc = *desc;
+    std::vector<rt_bvh_node> built;
+    const bool rebuild = ctx->tuning.tree == RT_TREE_DEVICE;
+    if (rebuild) {
+        int rc = device_tree(ctx, desc->triangles, desc->num_triangles, built);
+        if (rc) return rc;
+        packsrc.nodes = built.data();
+        packsrc.num_nodes = (uint32_t)built.size();
+        if (!desc->nodes || desc->num_nodes == 0) user = packsrc;
+    }
+    SceneCheck chk, pchk;
     try {
-        chk = check_scene(*desc);
+        chk = check_scene(user);
+        pchk = rebuild && !chk.error.empty() ? chk : (rebuild ? check_scene(packsrc) : chk);
     } catch (const std::exception& e) {
         return set_error(ctx, RT_ERR_NOMEM, std::string("rt_scene_upload: ") + e.what());
     }
     if (!chk.error.empty()) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: " + chk.error);
-    RT_HIP(ctx, hipSetDevice(ctx->device));
+    if (!pchk.error.empty()) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: device tree: " + pchk.error);
 
     PackedTree packed;
-    bool fast = chk.standard && chk.enclosing;
+    bool fast = pchk.standard && pchk.enclosing;
     if (fast) {
         try {
-            pack_tree(*desc, packed);
+            pack_tree(packsrc, packed);
         } catch (const std::exception& e) {
             return set_error(ctx, RT_ERR_NOMEM, std::string("rt_scene_upload: ") + e.what());
         }
         if (packed.height > kMaxPackedHeight) fast = false;
     }
+    desc = &user;
 
     rt_scene* s = new rt_scene();
     s->ctx = ctx;

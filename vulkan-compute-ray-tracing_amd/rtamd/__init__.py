@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "librtamd.so")
 RT_OK, RT_ERR_INVALID, RT_ERR_HIP, RT_ERR_NOMEM, RT_ERR_UNSUPPORTED, RT_ERR_BUSY, RT_ERR_CANCELLED, RT_ERR_NOT_FOUND = range(8)
 RT_MAT_LIGHT, RT_MAT_LAMBERTIAN, RT_MAT_METAL, RT_MAT_GLASS = range(4)
 SPHERES_OFF, SPHERES_NEAREST, SPHERES_SCENE = range(3)
+TREE_GIVEN, TREE_DEVICE = range(2)
 TRAVERSE_AUTO, TRAVERSE_PACKED, TRAVERSE_REFERENCE = range(3)
 LAYOUT_AUTO, LAYOUT_F32, LAYOUT_Q16, LAYOUT_W4 = range(4)
 
@@ -72,7 +73,7 @@ class RenderParams(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("node_layout", ctypes.c_uint32), ("refill", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("stack_lds", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("tree", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class Ubo(ctypes.Structure):
@@ -270,10 +271,10 @@ def _check(rc: int, ctx=None, what: str = ""):
 class HostScene:
     """Scene arrays in the reference's layouts (numpy)."""
 
-    def __init__(self, triangles, materials, nodes, lights=None, spheres=None):
+    def __init__(self, triangles, materials, nodes=None, lights=None, spheres=None):
         self.triangles = np.ascontiguousarray(triangles, dtype=TRI_DT)
         self.materials = np.ascontiguousarray(materials, dtype=MAT_DT)
-        self.nodes = np.ascontiguousarray(nodes, dtype=NODE_DT)
+        self.nodes = np.ascontiguousarray(nodes if nodes is not None else np.zeros(0, NODE_DT), dtype=NODE_DT)
         self.lights = np.ascontiguousarray(lights if lights is not None else np.zeros(0, LIGHT_DT), dtype=LIGHT_DT)
         self.spheres = np.ascontiguousarray(spheres if spheres is not None else np.zeros(0, SPHERE_DT), dtype=SPHERE_DT)
 
@@ -459,8 +460,8 @@ class Context:
     def check(self, rc, what=""):
         _check(rc, self._h, what)
 
-    def set_tuning(self, node_layout=0, refill=0, blocks_per_cu=0, stack_lds=0):
-        t = Tuning(node_layout, refill, blocks_per_cu, stack_lds)
+    def set_tuning(self, node_layout=0, refill=0, blocks_per_cu=0, stack_lds=0, tree=0):
+        t = Tuning(node_layout, refill, blocks_per_cu, stack_lds, tree)
         self.check(lib().rt_tuning_set(self._h, ctypes.byref(t)), "rt_tuning_set")
 
     def get_tuning(self) -> Tuning:
