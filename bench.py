@@ -290,6 +290,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="only the plain-kernel measurement (no render, device build, small and medium "
+                         "workloads, batched or served runs); for counter passes")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
@@ -301,6 +304,8 @@ def main():
                     help="after the measurement, copy a buffer of this many MiB once (a known-size kernel for "
                          "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.plain_only:
+        args.no_render = args.no_batched = args.no_served = True
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard by batch with no "
               "exchange; see the design notes). Run with --gpus 1.", file=sys.stderr)
@@ -402,8 +407,9 @@ def main():
                          "secondary_trace_mrays_per_s": round(sec / st.trace_secondary_ms / 1e3, 1)}
         log(f"render: {rms:.1f} ms, secondary traversal {out['render']['secondary_trace_mrays_per_s']} Mrays/s")
 
-    out["device_build"] = bench_device_build(ctx, hs, rays, threads, layout, node_bytes)
-    out.update(bench_small_and_medium(ctx, threads))
+    if not args.plain_only:
+        out["device_build"] = bench_device_build(ctx, hs, rays, threads, layout, node_bytes)
+        out.update(bench_small_and_medium(ctx, threads))
 
     if not args.no_batched:
         out["batched"] = bench_batched(ctx, threads, 5)

@@ -19,8 +19,10 @@
 //     positions i and j, 64 + that of the positions when the codes are
 //     equal; internal node i covers the sorted range with one end at i and
 //     splits it where the prefix shared with i ends
-//  6. boxes bottom-up: every leaf's thread climbs, stopping at a parent it
-//     reaches first; the second child to arrive writes the parent's box
+//  6. boxes: leaf records, a 64-ary pyramid of box unions over the sorted
+//     leaves, then every internal node unions the pyramid entries that tile
+//     its leaf range (no thread waits on another, so no cross-XCD cache
+//     maintenance is needed)
 // Numbering: internal node k -> node k, the leaf of sorted position k ->
 // node n-1+k.  One triangle gives a single leaf at node 0.
 #include <rocprim/device/device_radix_sort.hpp>
@@ -170,7 +172,7 @@ __device__ inline int delta(const uint64_t* __restrict__ keys, int64_t n, int64_
 }
 
 __global__ __launch_bounds__(256) void k_radix_tree(const uint64_t* __restrict__ keys, uint32_t n,
-                                                    rt_bvh_node* __restrict__ nodes, int32_t* __restrict__ parent) {
+                                                    rt_bvh_node* __restrict__ nodes, int32_t* __restrict__ far) {
     const int64_t N = n;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N - 1; i += (int64_t)gridDim.x * blockDim.x) {
         const int d = delta(keys, N, i, i + 1) - delta(keys, N, i, i - 1) >= 0 ? 1 : -1;
@@ -198,9 +200,7 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint64_t* __restrict__
         nd.rightNodeIndex = right;
         nd.objectIndex = -1;
         nd._pad1[0] = nd._pad1[1] = 0;
-        parent[left] = (int32_t)i;
-        parent[right] = (int32_t)i;
-        if (i == 0) parent[0] = -1;
+        far[i] = (int32_t)j;                  // node i covers sorted leaves [min(i,j), max(i,j)]
     }
 }
 
@@ -213,56 +213,100 @@ __device__ inline void store_box(rt_bvh_node* nd, const Box& b) {
     nd->max[2] = b.hi[2];
 }
 
-__device__ inline Box load_box(const rt_bvh_node* nd) {
-    Box b;
-    for (int a = 0; a < 3; ++a) {
-        b.lo[a] = nd->min[a];
-        b.hi[a] = nd->max[a];
-    }
-    return b;
-}
-
-// Leaves, then the climb.  `arrivals` holds one zeroed counter per internal node.
-__global__ __launch_bounds__(256) void k_boxes(const rt_triangle* __restrict__ tris, const uint32_t* __restrict__ order,
-                                               uint32_t n, rt_bvh_node* nodes, const int32_t* __restrict__ parent,
-                                               uint32_t* arrivals) {
+// Leaf records: the leaf of sorted position k is node n-1+k.
+__global__ __launch_bounds__(256) void k_leaves(const rt_triangle* __restrict__ tris, const uint32_t* __restrict__ order,
+                                                uint32_t n, rt_bvh_node* __restrict__ nodes) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t tri = order[k];
-        const int32_t leaf = (int32_t)(n - 1 + k);
-        Box b = tri_box(tris, tri);
-        rt_bvh_node* nd = nodes + leaf;
-        store_box(nd, b);
+        rt_bvh_node* nd = nodes + (n - 1 + k);
+        store_box(nd, tri_box(tris, tri));
         nd->_pad0 = 0.0f;
         nd->leftNodeIndex = -1;
         nd->rightNodeIndex = -1;
         nd->objectIndex = (int32_t)tri;
         nd->_pad1[0] = nd->_pad1[1] = 0;
-        // a radix tree over 95-bit keys is at most 96 levels deep; the step
-        // bound and the index checks only guard against a malformed tree
-        const uint32_t n_nodes = 2 * n - 1;
-        int32_t p = n > 1 ? parent[leaf] : -1;
-        for (int steps = 0; p >= 0 && (uint32_t)p < n - 1 && steps < 128; ++steps) {
-            // release this subtree's boxes, acquire the sibling's
-            const uint32_t before =
-                __hip_atomic_fetch_add(&arrivals[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (before == 0) break;           // the sibling's thread finishes this node
-            rt_bvh_node* pn = nodes + p;
-            const int32_t li = pn->leftNodeIndex, ri = pn->rightNodeIndex;
-            if ((uint32_t)li >= n_nodes || (uint32_t)ri >= n_nodes) break;
-            const Box l = load_box(nodes + li), r = load_box(nodes + ri);
-            Box u;
-            for (int a = 0; a < 3; ++a) {
-                u.lo[a] = fmin_(l.lo[a], r.lo[a]);
-                u.hi[a] = fmax_(l.hi[a], r.hi[a]);
-            }
-            store_box(pn, u);
-            p = parent[p];
-        }
     }
 }
 
-__global__ void k_fill_words(uint32_t* p, uint32_t value, uint32_t count) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = value;
+// Box pyramid over the sorted leaves: level 0 is the leaf records, entry m
+// of level L+1 is the union of entries [64m, 64m+64) of level L.  Every
+// internal node's box is then the union of at most ~126 entries per level
+// over its leaf range -- no thread waits on another.
+constexpr int kPyramidLevels = 6;             // 64^5 > 2^30 leaves
+struct PBox {
+    float4 lo, hi;
+};
+struct Pyramid {
+    const rt_bvh_node* leaves;                // level 0: nodes + n - 1
+    const PBox* lvl[kPyramidLevels];          // levels 1.. (lvl[0] unused)
+    uint32_t count[kPyramidLevels];
+    uint32_t levels;                          // levels present, level 0 included
+};
+
+__device__ inline Box pyramid_get(const Pyramid& p, uint32_t level, uint32_t k) {
+    Box b;
+    if (level == 0) {
+        const float4* q = reinterpret_cast<const float4*>(p.leaves + k);
+        const float4 lo = q[0], hi = q[1];
+        b.lo[0] = lo.x; b.lo[1] = lo.y; b.lo[2] = lo.z;
+        b.hi[0] = hi.x; b.hi[1] = hi.y; b.hi[2] = hi.z;
+    } else {
+        const PBox e = p.lvl[level][k];
+        b.lo[0] = e.lo.x; b.lo[1] = e.lo.y; b.lo[2] = e.lo.z;
+        b.hi[0] = e.hi.x; b.hi[1] = e.hi.y; b.hi[2] = e.hi.z;
+    }
+    return b;
+}
+
+__device__ inline void grow(Box& u, const Box& b) {
+    for (int a = 0; a < 3; ++a) {
+        u.lo[a] = fmin_(u.lo[a], b.lo[a]);
+        u.hi[a] = fmax_(u.hi[a], b.hi[a]);
+    }
+}
+
+// One wave per 64-entry block of level `level` -> entry of level+1.
+__global__ __launch_bounds__(256) void k_pyramid(Pyramid p, uint32_t level, PBox* __restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t blocks = p.count[level + 1];
+    for (uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) / 64u; m < blocks; m += gridDim.x * blockDim.x / 64u) {
+        const uint32_t k = m * 64u + lane;
+        Box b;
+        if (k < p.count[level]) {
+            b = pyramid_get(p, level, k);
+        } else {
+            for (int a = 0; a < 3; ++a) { b.lo[a] = INFINITY; b.hi[a] = -INFINITY; }
+        }
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = wave_min(b.lo[a]);
+            b.hi[a] = wave_max(b.hi[a]);
+        }
+        if (lane == 0) out[m] = PBox{make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f), make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f)};
+    }
+}
+
+// Internal node i's box: the union over its sorted leaf range.
+__global__ __launch_bounds__(256) void k_internal_boxes(Pyramid p, uint32_t n, const int32_t* __restrict__ far,
+                                                        rt_bvh_node* __restrict__ nodes) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n; i += gridDim.x * blockDim.x) {
+        const uint32_t j = (uint32_t)far[i];
+        uint32_t lo = i < j ? i : j, hi = (i < j ? j : i) + 1;    // [lo, hi) at level 0
+        Box u;
+        for (int a = 0; a < 3; ++a) { u.lo[a] = INFINITY; u.hi[a] = -INFINITY; }
+        for (uint32_t level = 0; lo < hi; ++level) {
+            if (level + 1 == p.levels || hi - lo <= 64u) {
+                for (uint32_t k = lo; k < hi; ++k) grow(u, pyramid_get(p, level, k));
+                break;
+            }
+            const uint32_t a = min(hi, (lo + 63u) & ~63u);          // head up to a block boundary
+            const uint32_t b = max(a, hi & ~63u);                    // tail from the last boundary
+            for (uint32_t k = lo; k < a; ++k) grow(u, pyramid_get(p, level, k));
+            for (uint32_t k = b; k < hi; ++k) grow(u, pyramid_get(p, level, k));
+            lo = a / 64u;                                           // whole blocks [a, b) one level up
+            hi = b / 64u;
+        }
+        store_box(nodes + i, u);
+    }
 }
 
 uint32_t grid(uint64_t work, uint32_t cap) {
@@ -290,7 +334,14 @@ extern "C" int rt_bvh_build_device(rt_ctx* ctx, const rt_triangle* tris_dev, uin
     size_t sort_bytes = 0;
     RT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
                                           (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0u, 63u, s));
-    const size_t n_int = n > 1 ? n - 1 : 1;
+    // pyramid level sizes
+    Pyramid pyr{};
+    pyr.count[0] = n;
+    pyr.levels = 1;
+    while (pyr.levels < (uint32_t)kPyramidLevels && pyr.count[pyr.levels - 1] > 1) {
+        pyr.count[pyr.levels] = (pyr.count[pyr.levels - 1] + 63u) / 64u;
+        ++pyr.levels;
+    }
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
     const size_t o_partial = take(sizeof(float) * 6 * kBoundsBlocks);
@@ -299,8 +350,9 @@ extern "C" int rt_bvh_build_device(rt_ctx* ctx, const rt_triangle* tris_dev, uin
     const size_t o_keys2 = take(sizeof(uint64_t) * n);
     const size_t o_vals = take(sizeof(uint32_t) * n);
     const size_t o_vals2 = take(sizeof(uint32_t) * n);
-    const size_t o_parent = take(sizeof(int32_t) * (2 * (size_t)n - 1));
-    const size_t o_arrive = take(sizeof(uint32_t) * n_int);
+    const size_t o_far = take(sizeof(int32_t) * n);
+    size_t o_lvl[kPyramidLevels] = {};
+    for (uint32_t L = 1; L < pyr.levels; ++L) o_lvl[L] = take(sizeof(PBox) * pyr.count[L]);
     const size_t o_sort = take(sort_bytes);
     int rc = ensure_workspace(ctx, kWsBuild, off);
     if (rc) return rc;
@@ -311,9 +363,10 @@ extern "C" int rt_bvh_build_device(rt_ctx* ctx, const rt_triangle* tris_dev, uin
     uint64_t* keys2 = (uint64_t*)(ws + o_keys2);
     uint32_t* vals = (uint32_t*)(ws + o_vals);
     uint32_t* vals2 = (uint32_t*)(ws + o_vals2);
-    int32_t* parent = (int32_t*)(ws + o_parent);
-    uint32_t* arrive = (uint32_t*)(ws + o_arrive);
+    int32_t* far = (int32_t*)(ws + o_far);
     void* sort_tmp = ws + o_sort;
+    pyr.leaves = nodes_dev + (n - 1);
+    for (uint32_t L = 1; L < pyr.levels; ++L) pyr.lvl[L] = (const PBox*)(ws + o_lvl[L]);
 
     const uint32_t cap = (uint32_t)ctx->num_cus * 16u;
     const uint32_t nb = grid(n, kBoundsBlocks);
@@ -324,16 +377,21 @@ extern "C" int rt_bvh_build_device(rt_ctx* ctx, const rt_triangle* tris_dev, uin
     hipLaunchKernelGGL(k_morton, dim3(grid(n, cap)), dim3(256), 0, s, tris_dev, n, bounds, keys, vals);
     RT_HIP(ctx, hipGetLastError());
     RT_HIP(ctx, rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys, keys2, vals, vals2, n, 0u, 63u, s));
-    hipLaunchKernelGGL(k_fill_words, dim3(grid(n_int, cap)), dim3(256), 0, s, arrive, 0u, (uint32_t)n_int);
-    RT_HIP(ctx, hipGetLastError());
-    hipLaunchKernelGGL(k_fill_words, dim3(grid(2 * (uint64_t)n - 1, cap)), dim3(256), 0, s, (uint32_t*)parent,
-                       0xffffffffu, 2 * n - 1);
-    RT_HIP(ctx, hipGetLastError());
     if (n > 1) {
-        hipLaunchKernelGGL(k_radix_tree, dim3(grid(n - 1, cap)), dim3(256), 0, s, keys2, n, nodes_dev, parent);
+        hipLaunchKernelGGL(k_radix_tree, dim3(grid(n - 1, cap)), dim3(256), 0, s, keys2, n, nodes_dev, far);
         RT_HIP(ctx, hipGetLastError());
     }
-    hipLaunchKernelGGL(k_boxes, dim3(grid(n, cap)), dim3(256), 0, s, tris_dev, vals2, n, nodes_dev, parent, arrive);
+    hipLaunchKernelGGL(k_leaves, dim3(grid(n, cap)), dim3(256), 0, s, tris_dev, vals2, n, nodes_dev);
     RT_HIP(ctx, hipGetLastError());
+    for (uint32_t L = 0; L + 1 < pyr.levels; ++L) {
+        // one wave (64 threads) per output entry
+        hipLaunchKernelGGL(k_pyramid, dim3(grid((uint64_t)pyr.count[L + 1] * 64u, cap)), dim3(256), 0, s, pyr, L,
+                           (PBox*)(ws + o_lvl[L + 1]));
+        RT_HIP(ctx, hipGetLastError());
+    }
+    if (n > 1) {
+        hipLaunchKernelGGL(k_internal_boxes, dim3(grid(n - 1, cap)), dim3(256), 0, s, pyr, n, far, nodes_dev);
+        RT_HIP(ctx, hipGetLastError());
+    }
     return RT_OK;
 }
