@@ -262,11 +262,19 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
         // ---- one traversal step per busy lane
         if (busy) {
             bool next = false;          // cur holds the next record to read
-            if (sp >= 0) {
-                if (cur >= 0 && LAYOUT == kLayoutW4) {
-                    const uint4* np = reinterpret_cast<const uint4*>(l_w4 + cur);
-                    uint4 a = np[0], b = np[1], c = np[2];
-                    int4 d = reinterpret_cast<const int4*>(np)[3];
+            if (sp >= 0 && LAYOUT == kLayoutW4) {
+                // One memory round trip per step whatever the lanes' mix of
+                // nodes and triangles: every lane issues its three 16-byte
+                // loads (node words 0-2 or the triangle record) and a fourth
+                // (node refs, or the triangle's first word again, so it never
+                // reads past the array) before any lane branches.
+                const bool node = cur >= 0;
+                const uint4* np = node ? reinterpret_cast<const uint4*>(l_w4 + cur)
+                                       : reinterpret_cast<const uint4*>(l_tris + ~cur);
+                const uint4 a = np[0], b = np[1], c = np[2];
+                const uint4 dw = np[node ? 3 : 0];
+                if (node) {
+                    const int4 d = make_int4((int)dw.x, (int)dw.y, (int)dw.z, (int)dw.w);
                     if (STATS) nv++;
                     float k0 = __builtin_inff(), k1 = __builtin_inff(), k2 = __builtin_inff(), k3 = __builtin_inff();
                     float tn;
@@ -294,7 +302,22 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                         cur = v0;
                         next = true;
                     }
-                } else if (cur >= 0) {
+                } else {
+                    const int32_t slot = ~cur;
+                    const PTri tr{make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
+                                              __uint_as_float(a.w)),
+                                  make_float4(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z),
+                                              __uint_as_float(b.w)),
+                                  make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z),
+                                              __uint_as_float(c.w))};
+                    if (STATS) nt++;
+                    float t, u, v;
+                    if (tri_test(tr, r.o, r.d, r.tmin, best_t, t, u, v)) {
+                        best_t = t; bu = u; bv = v; best = slot;
+                    }
+                }
+            } else if (sp >= 0) {
+                if (cur >= 0) {
                     bool h0, h1;
                     float tn0, tn1;
                     int32_t c0, c1;
