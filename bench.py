@@ -10,7 +10,9 @@ sphere), all resident in HBM before the timed region.
 Prints one JSON line: throughput in Mrays/s (one ray = one hit_bvh
 traversal), the roofline of the traversal kernel (algorithmic bytes = node
 records x 64 B (4-wide; 32 B with --layout q16) + triangle records x 48 B + ray 32 B +
-hit 16 B, counted per ray by an untimed pass of the counting kernel), the
+hit 16 B, counted per ray by an untimed pass of the counting kernel, over
+the traversal kernel's own time from stage events on its stream; `value`
+counts the whole call, the ray sort included), the
 CPU baseline (the
 oracle's restatement of hit_bvh in the shader's visiting order, on a bounded
 sample of the same rays), and side measurements:
@@ -359,7 +361,17 @@ def main():
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     log(f"plain: {ms_per_step:.3f} ms per step")
-    kernel_ms = sum(a.elapsed_time(b) for a, b in evs) / args.steps
+    call_ms = sum(a.elapsed_time(b) for a, b in evs) / args.steps
+    # the traversal kernel alone: stage events around its launch, on its stream
+    ctx.set_render_timing(True)
+    stages = []
+    for _ in range(max(3, min(args.steps, 10))):
+        ctx.closest_hit(scene, rays, hits)
+        stages.append(ctx.last_query_stats())
+    ctx.set_render_timing(False)
+    kernel_ms = sum(st.trace_ms for st in stages) / len(stages)
+    order_ms = sum(st.order_ms for st in stages) / len(stages)
+    sorted_rays = bool(stages[0].sorted)
     value = n / (ms_per_step / 1e3) / 1e6
     achieved = bytes_per_launch / (kernel_ms / 1e3) / 1e9
     traffic = read_traffic()
@@ -372,10 +384,13 @@ def main():
         "config": {"workload": f"closest-hit query, clustered soup {len(hs.triangles)} triangles in 4096 Gaussian "
                                f"clusters, {n} incoherent rays (origins U[-4,4]^3, uniform directions)",
                    "triangles": len(hs.triangles), "rays": n,
-                   "traversal": f"ordered, persistent, {args.layout} {node_bytes}-byte node records"},
+                   "traversal": f"ordered, persistent, {args.layout} {node_bytes}-byte node records",
+                   "trace_order": "origin Morton order (sorted on the device inside the timed call)"
+                   if sorted_rays else "index order"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_trace_packed", "kernel_ms": round(kernel_ms, 4),
+                     "call_ms": round(call_ms, 4), "order_ms": round(order_ms, 4),
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "stream_copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
         "visits": {"node_records_per_ray": round(v_node / n, 2), "tri_records_per_ray": round(v_tri / n, 3)},

@@ -193,6 +193,12 @@ enum { RT_LAYOUT_AUTO = 0, RT_LAYOUT_F32 = 1, RT_LAYOUT_Q16 = 2, RT_LAYOUT_W4 = 
  *                 caller's nodes, and a scene given without nodes uses the
  *                 device tree for both */
 enum { RT_TREE_GIVEN = 0, RT_TREE_DEVICE = 1 };
+/* Trace order of rt_closest_hit's rays (results always land at each ray's
+ * own index): RT_ORDER_AUTO sorts batches of 2^17 rays or more by the
+ * Morton code of their origins in the scene's box (8 bits per axis), so the
+ * lanes of a wave start near each other and share node fetches;
+ * RT_ORDER_GIVEN traces them in index order; RT_ORDER_SORTED always sorts. */
+enum { RT_ORDER_AUTO = 0, RT_ORDER_GIVEN = 1, RT_ORDER_SORTED = 2 };
 typedef struct rt_tuning {
     uint32_t node_layout;   /* RT_LAYOUT_*: binary 64-byte fp32 or 32-byte 16-bit
                                nodes, or 4-wide 64-byte 16-bit nodes            */
@@ -202,7 +208,8 @@ typedef struct rt_tuning {
                                8 or 16 = that many top entries in LDS, deeper ones
                                spill to HBM (more waves fit per CU)               */
     uint32_t tree;          /* RT_TREE_*                                          */
-    uint32_t reserved[3];
+    uint32_t ray_order;     /* RT_ORDER_*: the order a query's rays are traced in */
+    uint32_t reserved[2];
 } rt_tuning;
 int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t);
 int rt_tuning_get(const rt_ctx* ctx, rt_tuning* t);
@@ -286,13 +293,16 @@ int rt_render_frames(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo,
  * (one count per hit_bvh call, i.e. per ray). */
 int rt_render_last_rays(const rt_ctx* ctx, uint64_t* primary, uint64_t* secondary);
 
-/* Stage timing of renders (no counterpart in the reference, whose frame
- * loop has no timers).  With timing on, each render records HIP events
- * around every bounce's traversal kernel; rt_render_last_stats waits for
- * the last render and splits its time into the bounce-0 traversal, the
+/* Stage timing of renders and queries (no counterpart in the reference,
+ * whose frame loop has no timers).  With timing on, each render records HIP
+ * events around every bounce's traversal kernel; rt_render_last_stats waits
+ * for the last render and splits its time into the bounce-0 traversal, the
  * traversals of bounces 1.. (all secondary, incoherent rays) and the rest
- * (generate, shade, store).  Without timing the *_ms fields are 0 and
- * `timed` is 0. */
+ * (generate, shade, store).  Each rt_closest_hit records events around its
+ * ordering (key kernel and sort), its traversal (cursor reset and the
+ * traversal kernel) and its sphere pass; rt_query_last_stats waits for the
+ * last query and returns those times.  Without timing the *_ms fields are 0
+ * and `timed` is 0. */
 typedef struct rt_render_stats {
     uint64_t primary_rays, secondary_rays;
     uint32_t bounces, timed;
@@ -300,6 +310,11 @@ typedef struct rt_render_stats {
 } rt_render_stats;
 int rt_render_set_timing(rt_ctx* ctx, int enable);
 int rt_render_last_stats(rt_ctx* ctx, rt_render_stats* out);
+typedef struct rt_query_stats {
+    uint32_t timed, sorted;   /* sorted: the rays were traced in origin order */
+    float order_ms, trace_ms, spheres_ms, total_ms;
+} rt_query_stats;
+int rt_query_last_stats(rt_ctx* ctx, rt_query_stats* out);
 
 /* -------------------------------------------------- frame loop and display */
 /* The reference's main loop around the compute pass (src/main.cpp:166-183

@@ -51,7 +51,7 @@ def test_parameter_structs_match_the_header(rt):
     """ctypes mirrors of the C-ABI's parameter structs (include/rtamd.h)."""
     import ctypes
     sizes = {"Tuning": 32, "QueryParams": 16, "RenderParams": 32, "RenderStats": 40, "BatchItem": 32,
-             "BatchConfig": 32, "ServerConfig": 40, "SceneInfo": 48}
+             "BatchConfig": 32, "ServerConfig": 40, "SceneInfo": 48, "QueryStats": 24}
     for name, size in sizes.items():
         assert ctypes.sizeof(getattr(rt, name)) == size, name
     assert rt.Tuning.tree.offset == 16 and rt.RenderStats.total_ms.offset == 24

@@ -240,3 +240,49 @@ def test_invalid_scene_rejected_by_upload(rt, gpu_ctx):
     with pytest.raises(rtamd.RtError) as e:
         gpu_ctx.upload(hs)
     assert e.value.status == rtamd.RT_ERR_INVALID and "child index" in str(e.value)
+
+
+def test_trace_order_is_invisible(rt, gpu_ctx):
+    """RT_ORDER_SORTED traces the rays in Morton order of their origins:
+    hits and per-ray counters are byte-identical to index order."""
+    hs = rt.scene_clusters(1 << 18, 21)
+    scene = gpu_ctx.upload(hs)
+    rays = rt.rays_to_device(rt.rays_box(300000, 22, 4.0))
+    print("case: 300k rays, sorted vs given trace order", flush=True)
+    out = {}
+    for mode in (rt.ORDER_GIVEN, rt.ORDER_SORTED, rt.ORDER_AUTO):
+        gpu_ctx.set_tuning(ray_order=mode)
+        try:
+            h = rt.hits_from_device(gpu_ctx.closest_hit(scene, rays))
+            _, nv, tt = gpu_ctx.closest_hit_stats(scene, rays)
+            torch.cuda.synchronize()
+            out[mode] = (h, nv.cpu().numpy(), tt.cpu().numpy())
+        finally:
+            gpu_ctx.set_tuning()
+    for mode in (rt.ORDER_SORTED, rt.ORDER_AUTO):
+        assert np.array_equal(out[mode][0].view(np.uint8), out[rt.ORDER_GIVEN][0].view(np.uint8))
+        assert np.array_equal(out[mode][1], out[rt.ORDER_GIVEN][1])
+        assert np.array_equal(out[mode][2], out[rt.ORDER_GIVEN][2])
+
+
+def test_query_stage_timing(rt, gpu_ctx):
+    hs = rt.scene_clusters(1 << 18, 23)
+    scene = gpu_ctx.upload(hs)
+    rays = rt.rays_to_device(rt.rays_box(200000, 24, 4.0))
+    print("case: query stage timing", flush=True)
+    gpu_ctx.set_render_timing(True)
+    got = {}
+    try:
+        for mode, sorted_ in ((rt.ORDER_SORTED, 1), (rt.ORDER_GIVEN, 0)):
+            gpu_ctx.set_tuning(ray_order=mode)
+            gpu_ctx.closest_hit(scene, rays)
+            st = gpu_ctx.last_query_stats()
+            assert st.timed == 1 and st.sorted == sorted_ and st.trace_ms > 0
+            assert st.order_ms + st.trace_ms + st.spheres_ms <= st.total_ms * (1 + 1e-5) + 1e-3
+            got[sorted_] = st.order_ms
+        assert got[1] > got[0]                      # the key kernel and the sort take time
+    finally:
+        gpu_ctx.set_render_timing(False)
+        gpu_ctx.set_tuning()
+    gpu_ctx.closest_hit(scene, rays)
+    assert gpu_ctx.last_query_stats().timed == 0

@@ -88,6 +88,8 @@ extern "C" int rt_destroy(rt_ctx* ctx) {
     if (ctx->d_cursors) (void)hipFree(ctx->d_cursors);
     for (hipEvent_t e : ctx->render_ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->query_ev)
+        if (e) (void)hipEventDestroy(e);
     delete ctx;
     return RT_OK;
 }
@@ -97,6 +99,7 @@ extern "C" int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t) {
     if (t->node_layout > RT_LAYOUT_W4) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: unknown node layout");
     if (t->refill > 64) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: refill must be 0..64");
     if (t->tree > RT_TREE_DEVICE) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: unknown tree");
+    if (t->ray_order > RT_ORDER_SORTED) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: unknown ray order");
     if (t->stack_lds != 0 && t->stack_lds != 8 && t->stack_lds != 16)
         return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: stack_lds must be 0, 8 or 16");
     ctx->tuning = *t;
@@ -135,15 +138,37 @@ int query_common(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays, uint32_
     bool packed = p.traversal == RT_TRAVERSE_PACKED ||
                   (p.traversal == RT_TRAVERSE_AUTO && scene->fast && p.max_stack == 0);
     if (nv || tt) packed = true;
+    if (packed && p.max_stack)
+        return set_error(ctx, RT_ERR_UNSUPPORTED, "rt_closest_hit: max_stack applies to reference traversal");
+    // stage timing: ev[0] start, ev[1] order ready, ev[2] traversal done, ev[3] spheres done
+    hipEvent_t* ev = ctx->time_render ? ctx->query_ev : nullptr;
+    ctx->last_query_timed = false;
+    ctx->last_query_sorted = false;
+    if (ev) {
+        for (int i = 0; i < 4; ++i)
+            if (!ev[i]) RT_HIP(ctx, hipEventCreate(&ev[i]));
+        RT_HIP(ctx, hipEventRecord(ev[0], s));
+    }
     int rc;
     if (packed) {
-        if (p.max_stack) return set_error(ctx, RT_ERR_UNSUPPORTED, "rt_closest_hit: max_stack applies to reference traversal");
+        rc = ray_order(ctx, a, s, &a.order);
+        if (rc) return rc;
+        ctx->last_query_sorted = a.order != nullptr;
+        if (ev) RT_HIP(ctx, hipEventRecord(ev[1], s));
         rc = launch_trace_packed(ctx, a, s);
     } else {
+        if (ev) RT_HIP(ctx, hipEventRecord(ev[1], s));
         rc = launch_trace_reference(ctx, a, p.max_stack, s);
     }
     if (rc) return rc;
-    return launch_sphere_pass(ctx, a, s);
+    if (ev) RT_HIP(ctx, hipEventRecord(ev[2], s));
+    rc = launch_sphere_pass(ctx, a, s);
+    if (rc) return rc;
+    if (ev) {
+        RT_HIP(ctx, hipEventRecord(ev[3], s));
+        ctx->last_query_timed = true;
+    }
+    return RT_OK;
 }
 
 }  // namespace
@@ -226,6 +251,25 @@ extern "C" int rt_render_last_stats(rt_ctx* ctx, rt_render_stats* out) {
     out->trace_primary_ms = ms(ev[1], ev[2]);
     for (uint32_t b = 1; b < B; ++b) out->trace_secondary_ms += ms(ev[1 + 2 * b], ev[2 + 2 * b]);
     out->shade_ms = out->total_ms - out->trace_primary_ms - out->trace_secondary_ms;
+    out->timed = 1;
+    return RT_OK;
+}
+
+extern "C" int rt_query_last_stats(rt_ctx* ctx, rt_query_stats* out) {
+    if (!ctx || !out) return RT_ERR_INVALID;
+    std::memset(out, 0, sizeof *out);
+    out->sorted = ctx->last_query_sorted ? 1u : 0u;
+    if (!ctx->last_query_timed) return RT_OK;
+    hipEvent_t* ev = ctx->query_ev;
+    RT_HIP(ctx, hipEventSynchronize(ev[3]));
+    auto ms = [&](hipEvent_t a, hipEvent_t b) -> float {
+        float t = 0.0f;
+        return hipEventElapsedTime(&t, a, b) == hipSuccess ? t : 0.0f;
+    };
+    out->order_ms = ms(ev[0], ev[1]);
+    out->trace_ms = ms(ev[1], ev[2]);
+    out->spheres_ms = ms(ev[2], ev[3]);
+    out->total_ms = ms(ev[0], ev[3]);
     out->timed = 1;
     return RT_OK;
 }

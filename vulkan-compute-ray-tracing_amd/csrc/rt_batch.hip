@@ -174,7 +174,7 @@ int launch_batch_kernel(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
     const uint32_t blocks = b->grid_blocks;
     if (!blocks) return RT_OK;
     int32_t* spill = RING ? b->d_spill : nullptr;
-    TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, b->d_cursor, b->total, b->cfg.refill};
+    TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, b->d_cursor, b->total, b->cfg.refill, nullptr};
     BatchIo bio{b->d_table, b->d_offsets, b->d_chunk_item, b->cfg.regroup ? b->d_perm : nullptr};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), 0, s, io, bio, spill);
     RT_HIP(ctx, hipGetLastError());
@@ -280,7 +280,7 @@ int launch_dyn_t(rt_ctx* ctx, const BatchIo& bio, uint32_t* cursor, uint32_t tot
     uint32_t need = (total + 255u) / 256u;
     if (need < blocks) blocks = need;
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, cursor, 0u, 1u);
-    TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, cursor, total, refill};
+    TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, cursor, total, refill, nullptr};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, io, bio, (int32_t*)nullptr);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;

@@ -179,8 +179,8 @@ struct rt_ctx {
     int num_cus = 0;
     // grow-only device workspaces: [0] render path state, [1] reference
     // traversal stacks, [2] statistics scratch, [3] traversal stack spill,
-    // [4] BVH builder scratch
-    static constexpr int kWorkspaces = 5;
+    // [4] BVH builder scratch, [5] ray-order keys and permutation
+    static constexpr int kWorkspaces = 6;
     void* ws[kWorkspaces] = {};
     size_t ws_bytes[kWorkspaces] = {};
     uint32_t* d_counters = nullptr;   // small zero-filled counter block
@@ -191,10 +191,14 @@ struct rt_ctx {
     // opt-in stage timing of renders: events around generate, each bounce's
     // trace and shade, and store (created on the first timed render)
     static constexpr int kRenderEvents = 2 * 63 + 2;
-    bool time_render = false;
+    bool time_render = false;           // stage timing of renders and queries
     hipEvent_t render_ev[kRenderEvents] = {};
     uint32_t last_bounces = 0;
     bool last_timed = false;
+    // query stage events: start, ordered, traced, spheres done
+    hipEvent_t query_ev[4] = {};
+    bool last_query_timed = false;
+    bool last_query_sorted = false;
 };
 
 namespace rtamd {
@@ -203,7 +207,7 @@ int set_error(rt_ctx* ctx, int code, const std::string& msg);
 // Triangles of one OBJ file (rt_obj.cpp); err receives the reason on failure.
 int obj_triangles(const char* path, uint32_t material, std::vector<rt_triangle>& tris, std::string& err);
 int hip_error(rt_ctx* ctx, hipError_t e, const char* what);
-enum { kWsRender = 0, kWsStack = 1, kWsStats = 2, kWsSpill = 3, kWsBuild = 4 };
+enum { kWsRender = 0, kWsStack = 1, kWsStats = 2, kWsSpill = 3, kWsBuild = 4, kWsOrder = 5 };
 constexpr uint32_t kCursorSlots = 1024;
 int ensure_workspace(rt_ctx* ctx, int slot, size_t bytes);
 
@@ -224,8 +228,14 @@ struct TraceArgs {
     uint32_t* node_visits;      // optional per-ray counters
     uint32_t* tri_tests;
     bool render = false;        // a bounce of the render's wavefront loop
+    const uint32_t* order = nullptr;   // packed traversal: trace order, or index order
 };
 int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
+// The trace order of a packed query (rt_trace.hip): sorts the rays by
+// origin cell into ctx's order workspace when ctx->tuning.ray_order asks
+// for it (or, by default, for large batches on large scenes); *order stays
+// null for index order.
+int ray_order(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, const uint32_t** order);
 int resolved_layout(const rt_ctx* ctx);   // RT_LAYOUT_* actually used
 int launch_trace_reference(rt_ctx* ctx, const TraceArgs& a, uint32_t max_stack, hipStream_t s);
 // Spheres after a traversal (rt_sphere.hip): no-op unless a.spheres is set

@@ -17,6 +17,8 @@
 //    unconditionally, right popped first, optional MAX_STACK_DEPTH guard.
 //    Used for trees that do not enclose their subtrees, for the 16-entry
 //    truncation mode, and as a tie-exact mode.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "rt_trace_dev.hpp"
 
 #pragma clang fp contract(off)
@@ -120,6 +122,42 @@ uint32_t* next_cursor(rt_ctx* ctx) {
     return ctx->d_cursors + k;
 }
 
+// Trace order of a query: rays sorted by the Morton code of their origin's
+// cell in a 256^3 grid over the scene's node bounds (the 16-bit frame), so
+// the lanes of a wave start near each other and fetch the same upper nodes.
+// Only the order changes: each ray is still traced alone and its hit lands
+// at its own index.
+constexpr uint32_t kOrderBits = 8;                 // cells per axis: 2^8
+constexpr uint32_t kOrderMinRays = 1u << 17;
+constexpr uint32_t kOrderMinTriangles = 1u << 20;
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {   // bit b -> bit 3b, b < 10
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t order_cell(float o, float org, float rcp) {
+    // (o - org) * rcp spans [0, 65000] over the bounds; 2^8 cells
+    const float q = (o - org) * rcp * (256.0f / 65000.0f);
+    return q >= 255.0f ? 255u : (q > 0.0f ? (uint32_t)q : 0u);
+}
+
+__global__ __launch_bounds__(256) void k_ray_keys(const rt_ray* __restrict__ rays, uint32_t n, QFrame f,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 o = reinterpret_cast<const float4*>(rays)[2 * (size_t)i];
+        keys[i] = (spread10(order_cell(o.x, f.ox, f.rx)) << 2) | (spread10(order_cell(o.y, f.oy, f.ry)) << 1) |
+                  spread10(order_cell(o.z, f.oz, f.rz));
+        idx[i] = i;
+    }
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 template <int STACK, int LAYOUT, int RING>
 int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
     const rt_scene* sc = a.scene;
@@ -151,7 +189,7 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
     if (refill > 64u) refill = 64u;
     TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->frame,
                   LAYOUT == kLayoutW4 ? sc->root4 : sc->root_ref};
-    TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill};
+    TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill, a.order};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, ts, io, spill);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;
@@ -172,6 +210,37 @@ int launch_w4(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
 }
 
 }  // namespace
+
+// The sorted order of a query's rays in ctx's order workspace, or null when
+// the batch keeps index order.
+int ray_order(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, const uint32_t** order) {
+    *order = nullptr;
+    const uint32_t mode = ctx->tuning.ray_order;
+    if (a.render || a.n_dev || mode == RT_ORDER_GIVEN || !a.scene->fast) return RT_OK;
+    // by default only where it pays: many rays, and a tree far larger than
+    // the caches (a small scene's nodes stay in L2 in any order)
+    if (mode == RT_ORDER_AUTO && (a.n < kOrderMinRays || a.scene->info.num_triangles < kOrderMinTriangles))
+        return RT_OK;
+    const uint32_t n = a.n;
+    size_t sort_bytes = 0;
+    RT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                          (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0u, 3u * kOrderBits, s));
+    const size_t vec = align256(sizeof(uint32_t) * (size_t)n);
+    int rc = ensure_workspace(ctx, kWsOrder, 4 * vec + align256(sort_bytes));
+    if (rc) return rc;
+    char* ws = (char*)ctx->ws[kWsOrder];
+    uint32_t* keys = (uint32_t*)ws;
+    uint32_t* keys2 = (uint32_t*)(ws + vec);
+    uint32_t* idx = (uint32_t*)(ws + 2 * vec);
+    uint32_t* perm = (uint32_t*)(ws + 3 * vec);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255u) / 256u, (uint64_t)ctx->num_cus * 16u);
+    hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(256), 0, s, a.rays, n, a.scene->frame, keys, idx);
+    RT_HIP(ctx, hipGetLastError());
+    RT_HIP(ctx, rocprim::radix_sort_pairs((void*)(ws + 4 * vec), sort_bytes, keys, keys2, idx, perm, n, 0u,
+                                          3u * kOrderBits, s));
+    *order = perm;
+    return RT_OK;
+}
 
 int resolved_layout(const rt_ctx* ctx) {
     uint32_t l = ctx->tuning.node_layout;

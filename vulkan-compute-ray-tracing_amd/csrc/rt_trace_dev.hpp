@@ -132,6 +132,7 @@ struct TraceIo {
     uint32_t* cursor;
     uint32_t n_host;
     uint32_t refill;
+    const uint32_t* order;    // trace order (ray indices), or null: index order
 };
 
 __device__ __forceinline__ float lo16(uint32_t w) { return (float)(w & 0xffffu); }
@@ -238,8 +239,8 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                         ray_i = bio.perm ? bio.perm[mine] : mine - e->base;
                         r = load_ray_stream(e->rays, ray_i);
                     } else {
-                        ray_i = mine;
-                        r = load_ray_stream(io.rays, mine);
+                        ray_i = io.order ? io.order[mine] : mine;
+                        r = load_ray_stream(io.rays, ray_i);
                     }
                     inv = f3{safe_inv(r.d.x), safe_inv(r.d.y), safe_inv(r.d.z)};
                     bo = r.o;

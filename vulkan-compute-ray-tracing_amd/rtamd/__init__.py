@@ -20,6 +20,7 @@ RT_OK, RT_ERR_INVALID, RT_ERR_HIP, RT_ERR_NOMEM, RT_ERR_UNSUPPORTED, RT_ERR_BUSY
 RT_MAT_LIGHT, RT_MAT_LAMBERTIAN, RT_MAT_METAL, RT_MAT_GLASS = range(4)
 SPHERES_OFF, SPHERES_NEAREST, SPHERES_SCENE = range(3)
 TREE_GIVEN, TREE_DEVICE = range(2)
+ORDER_AUTO, ORDER_GIVEN, ORDER_SORTED = range(3)
 TRAVERSE_AUTO, TRAVERSE_PACKED, TRAVERSE_REFERENCE = range(3)
 LAYOUT_AUTO, LAYOUT_F32, LAYOUT_Q16, LAYOUT_W4 = range(4)
 
@@ -73,7 +74,7 @@ class RenderParams(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("node_layout", ctypes.c_uint32), ("refill", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("stack_lds", ctypes.c_uint32),
-                ("tree", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("tree", ctypes.c_uint32), ("ray_order", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class Ubo(ctypes.Structure):
@@ -104,6 +105,14 @@ class RenderStats(ctypes.Structure):
     _fields_ = [("primary_rays", ctypes.c_uint64), ("secondary_rays", ctypes.c_uint64),
                 ("bounces", ctypes.c_uint32), ("timed", ctypes.c_uint32)] + \
                [(k, ctypes.c_float) for k in ("total_ms", "trace_primary_ms", "trace_secondary_ms", "shade_ms")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class QueryStats(ctypes.Structure):
+    _fields_ = [("timed", ctypes.c_uint32), ("sorted", ctypes.c_uint32)] + \
+               [(k, ctypes.c_float) for k in ("order_ms", "trace_ms", "spheres_ms", "total_ms")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -161,7 +170,7 @@ EXPORTS = [
     "rt_create", "rt_destroy", "rt_last_error", "rt_status_string", "rt_scene_upload", "rt_scene_info_get",
     "rt_scene_check", "rt_tuning_set", "rt_tuning_get",
     "rt_scene_release", "rt_closest_hit", "rt_closest_hit_stats", "rt_render", "rt_render_frames",
-    "rt_render_last_rays", "rt_render_set_timing", "rt_render_last_stats", "rt_bvh_build", "rt_bvh_build_seeded", "rt_bvh_build_device", "rt_scene_lights",
+    "rt_render_last_rays", "rt_render_set_timing", "rt_render_last_stats", "rt_query_last_stats", "rt_bvh_build", "rt_bvh_build_seeded", "rt_bvh_build_device", "rt_scene_lights",
     "rt_frame_loop_create", "rt_frame_loop_destroy", "rt_frame_loop_step", "rt_frame_loop_display",
     "rt_frame_loop_images", "rt_frame_loop_read", "rt_postprocess_gamma", "rt_gamma_table", "rt_image_write",
     "rt_batch_create", "rt_batch_set_config", "rt_batch_get_config", "rt_batch_launch", "rt_batch_autotune",
@@ -202,6 +211,7 @@ def lib():
         "rt_render_last_rays": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "rt_render_set_timing": (i32, [P, i32]),
         "rt_render_last_stats": (i32, [P, ctypes.POINTER(RenderStats)]),
+        "rt_query_last_stats": (i32, [P, ctypes.POINTER(QueryStats)]),
         "rt_batch_create": (i32, [P, ctypes.POINTER(BatchItem), u32, ctypes.POINTER(BatchConfig), ctypes.POINTER(P)]),
         "rt_batch_set_config": (i32, [P, P, ctypes.POINTER(BatchConfig)]),
         "rt_batch_get_config": (i32, [P, ctypes.POINTER(BatchConfig)]),
@@ -460,8 +470,8 @@ class Context:
     def check(self, rc, what=""):
         _check(rc, self._h, what)
 
-    def set_tuning(self, node_layout=0, refill=0, blocks_per_cu=0, stack_lds=0, tree=0):
-        t = Tuning(node_layout, refill, blocks_per_cu, stack_lds, tree)
+    def set_tuning(self, node_layout=0, refill=0, blocks_per_cu=0, stack_lds=0, tree=0, ray_order=0):
+        t = Tuning(node_layout, refill, blocks_per_cu, stack_lds, tree, ray_order)
         self.check(lib().rt_tuning_set(self._h, ctypes.byref(t)), "rt_tuning_set")
 
     def get_tuning(self) -> Tuning:
@@ -560,6 +570,11 @@ class Context:
 
     def set_render_timing(self, enable: bool):
         self.check(lib().rt_render_set_timing(self._h, 1 if enable else 0), "rt_render_set_timing")
+
+    def last_query_stats(self) -> QueryStats:
+        st = QueryStats()
+        self.check(lib().rt_query_last_stats(self._h, ctypes.byref(st)), "rt_query_last_stats")
+        return st
 
     def last_render_stats(self) -> RenderStats:
         st = RenderStats()
