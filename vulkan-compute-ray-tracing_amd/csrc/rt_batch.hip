@@ -6,8 +6,8 @@
 // lane that needs work takes the next numbers from one cursor (the same
 // per-lane refetch as the plain kernel) and finds its item through a
 // 64-ray chunk table.  Regrouping (optional) first bins each item's rays by
-// direction octant and a 4x4x4 origin cell in the scene's box frame, so
-// neighbouring lanes start in the same region heading the same way; the
+// the 16x16x16 origin cell (Morton order) in the scene's box frame, so
+// neighbouring lanes start in the same region and share node fetches; the
 // binning permutation only changes which lane traces a ray, never a result.
 // Every step is a kernel (counters are cleared by a fill kernel, not a
 // memset), so the sequence can be captured into a graph and replayed.
@@ -28,7 +28,7 @@
 namespace rtamd {
 namespace {
 
-constexpr uint32_t kBins = 512;          // 8 octants x 64 origin cells
+constexpr uint32_t kBins = 4096;         // 16^3 origin cells per item, Morton order
 constexpr uint32_t kChunk = 64;          // rays per chunk-table entry
 
 __device__ __forceinline__ uint32_t item_of(const BatchIo& bio, uint32_t g) {
@@ -44,16 +44,17 @@ __global__ __launch_bounds__(256) void k_regroup_bin(BatchIo bio, uint32_t total
         const uint32_t b = item_of(bio, g);
         const BatchEntry& e = bio.table[b];
         const uint32_t local = g - e.base;
-        const float4* rp = reinterpret_cast<const float4*>(e.rays) + 2 * (size_t)local;
-        float4 o = rp[0], d = rp[1];
-        uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
+        const float4 o = reinterpret_cast<const float4*>(e.rays)[2 * (size_t)local];
+        // (v - org) * rs spans [0, 65000] over the item's scene; 16 cells per axis
         auto cell = [](float v, float org, float rs) -> uint32_t {
-            float q = (v - org) * rs * (1.0f / 16384.0f);
-            return q <= 0.0f ? 0u : (q >= 3.0f ? 3u : (uint32_t)q);
+            const float q = (v - org) * rs * (16.0f / 65000.0f);
+            return q >= 15.0f ? 15u : (q > 0.0f ? (uint32_t)q : 0u);
         };
-        uint32_t c = cell(o.x, e.frame.ox, e.frame.rx) * 16u + cell(o.y, e.frame.oy, e.frame.ry) * 4u +
-                     cell(o.z, e.frame.oz, e.frame.rz);
-        uint32_t bin = oct * 64u + c;
+        auto spread4 = [](uint32_t x) -> uint32_t {   // bit k -> bit 3k
+            return (x & 1u) | ((x & 2u) << 2) | ((x & 4u) << 4) | ((x & 8u) << 6);
+        };
+        const uint32_t bin = (spread4(cell(o.x, e.frame.ox, e.frame.rx)) << 2) |
+                             (spread4(cell(o.y, e.frame.oy, e.frame.ry)) << 1) | spread4(cell(o.z, e.frame.oz, e.frame.rz));
         uint32_t rank = atomicAdd(&hist[b * kBins + bin], 1u);
         bin_rank[2 * (size_t)g] = bin;
         bin_rank[2 * (size_t)g + 1] = rank;
@@ -67,18 +68,26 @@ __global__ __launch_bounds__(256) void k_regroup_scan(uint32_t* __restrict__ his
     if (b >= n_items) return;
     uint32_t* h = hist + (size_t)b * kBins;
     const uint32_t t = threadIdx.x;
-    uint32_t a0 = h[2 * t], a1 = h[2 * t + 1];
-    part[t] = a0 + a1;
+    constexpr uint32_t kPer = kBins / 256;      // consecutive bins per thread
+    uint32_t v[kPer];
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < kPer; ++k) {
+        v[k] = h[t * kPer + k];
+        sum += v[k];
+    }
+    part[t] = sum;
     __syncthreads();
     for (uint32_t off = 1; off < 256; off <<= 1) {
-        uint32_t v = t >= off ? part[t - off] : 0u;
+        const uint32_t x = t >= off ? part[t - off] : 0u;
         __syncthreads();
-        part[t] += v;
+        part[t] += x;
         __syncthreads();
     }
-    uint32_t excl = part[t] - (a0 + a1);
-    h[2 * t] = excl;
-    h[2 * t + 1] = excl + a0;
+    uint32_t excl = part[t] - sum;
+    for (uint32_t k = 0; k < kPer; ++k) {
+        h[t * kPer + k] = excl;
+        excl += v[k];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_regroup_scatter(BatchIo bio, uint32_t total, const uint32_t* __restrict__ hist,

@@ -324,6 +324,12 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
         const uint32_t* n_dev = b == 0 ? nullptr : counters + b;
         TraceArgs ta{sc, rays[b & 1], P, n_dev, hits, prm.spheres, nullptr, nullptr, true};
         if (ev) RT_HIP(ctx, hipEventRecord(ev[1 + 2 * b], s));
+        // secondary bounces start on surfaces all over the scene: trace them
+        // in origin order (camera rays share one origin and are coherent)
+        if (b > 0 && use_packed) {
+            rc = ray_order(ctx, ta, s, &ta.order);
+            if (rc) return rc;
+        }
         rc = use_packed ? launch_trace_packed(ctx, ta, s) : launch_trace_reference(ctx, ta, prm.max_stack, s);
         if (rc) return rc;
         rc = launch_sphere_pass(ctx, ta, s);

@@ -156,6 +156,25 @@ __global__ __launch_bounds__(256) void k_ray_keys(const rt_ray* __restrict__ ray
     }
 }
 
+// The same keys for a render bounce's queue, whose length lives on the
+// device: slots at or past *n_dev get key 2^24, above every cell key, so the
+// live rays sort first.
+__global__ __launch_bounds__(256) void k_queue_keys(const rt_ray* __restrict__ rays, const uint32_t* __restrict__ n_dev,
+                                                    uint32_t cap, QFrame f, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ idx) {
+    const uint32_t n = *n_dev;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+        uint32_t k = 1u << (3 * kOrderBits);
+        if (i < n) {
+            const float4 o = reinterpret_cast<const float4*>(rays)[2 * (size_t)i];
+            k = (spread10(order_cell(o.x, f.ox, f.rx)) << 2) | (spread10(order_cell(o.y, f.oy, f.ry)) << 1) |
+                spread10(order_cell(o.z, f.oz, f.rz));
+        }
+        keys[i] = k;
+        idx[i] = i;
+    }
+}
+
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 template <int STACK, int LAYOUT, int RING>
@@ -216,15 +235,17 @@ int launch_w4(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
 int ray_order(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, const uint32_t** order) {
     *order = nullptr;
     const uint32_t mode = ctx->tuning.ray_order;
-    if (a.render || a.n_dev || mode == RT_ORDER_GIVEN || !a.scene->fast) return RT_OK;
+    if (mode == RT_ORDER_GIVEN || !a.scene->fast) return RT_OK;
     // by default only where it pays: many rays, and a tree far larger than
-    // the caches (a small scene's nodes stay in L2 in any order)
+    // the caches (a small scene's nodes stay in L2 in any order).  For a
+    // render queue `n` is its capacity.
     if (mode == RT_ORDER_AUTO && (a.n < kOrderMinRays || a.scene->info.num_triangles < kOrderMinTriangles))
         return RT_OK;
     const uint32_t n = a.n;
+    const uint32_t key_bits = 3u * kOrderBits + (a.n_dev ? 1u : 0u);
     size_t sort_bytes = 0;
     RT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                          (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0u, 3u * kOrderBits, s));
+                                          (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0u, key_bits, s));
     const size_t vec = align256(sizeof(uint32_t) * (size_t)n);
     int rc = ensure_workspace(ctx, kWsOrder, 4 * vec + align256(sort_bytes));
     if (rc) return rc;
@@ -234,10 +255,13 @@ int ray_order(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, const uint32_t** o
     uint32_t* idx = (uint32_t*)(ws + 2 * vec);
     uint32_t* perm = (uint32_t*)(ws + 3 * vec);
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255u) / 256u, (uint64_t)ctx->num_cus * 16u);
-    hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(256), 0, s, a.rays, n, a.scene->frame, keys, idx);
+    if (a.n_dev)
+        hipLaunchKernelGGL(k_queue_keys, dim3(grid), dim3(256), 0, s, a.rays, a.n_dev, n, a.scene->frame, keys, idx);
+    else
+        hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(256), 0, s, a.rays, n, a.scene->frame, keys, idx);
     RT_HIP(ctx, hipGetLastError());
     RT_HIP(ctx, rocprim::radix_sort_pairs((void*)(ws + 4 * vec), sort_bytes, keys, keys2, idx, perm, n, 0u,
-                                          3u * kOrderBits, s));
+                                          key_bits, s));
     *order = perm;
     return RT_OK;
 }
