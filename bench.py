@@ -128,7 +128,7 @@ def bench_batched(ctx, threads: int, reps: int):
     return out
 
 
-def bench_served(ctx, threads: int, seconds: float, fifo_seconds: float):
+def bench_served(ctx, threads: int, seconds: float, fifo_seconds: float, rates=()):
     """Poisson request stream against the batching server and the FIFO plain server."""
     import rtamd
     t0 = time.perf_counter()
@@ -163,6 +163,19 @@ def bench_served(ctx, threads: int, seconds: float, fifo_seconds: float):
     log(f"served: p50 {served['p50_ms']} ms p99 {served['p99_ms']} ms, {served['mrays_per_s']} Mrays/s")
     fifo = run(rtamd.SERVE_FIFO, 1, fifo_seconds, False)
     log(f"served: fifo p50 {fifo['p50_ms']} ms, {fifo['mrays_per_s']} Mrays/s")
+    capacity = []
+    for rate in rates:
+        srv = rtamd.Server(ctx, hbm_budget=budget)
+        for h in hss:
+            srv.add_scene(h)
+        r = srv.poisson(rate=float(rate), duration=6.0, render_fraction=0.1, swap_interval=3.0,
+                        swap_triangles=500000, seed=7, threads=threads)
+        srv.close()
+        st = r.stats
+        capacity.append({"rate_per_s": rate, "requests": int(r.requests), "p50_ms": round(st.p50_ms, 3),
+                         "p99_ms": round(st.p99_ms, 3), "wall_s": round(r.wall_s, 2),
+                         "mrays_per_s": round(st.rays / r.wall_s / 1e6, 1)})
+        log(f"served: {rate}/s p99 {capacity[-1]['p99_ms']} ms, {capacity[-1]['mrays_per_s']} Mrays/s")
     out = {"workload": "16 resident clustered-soup scenes, triangles log-uniform [10k,2M] (seeds 300-315), "
                        "HBM budget 1.05x their footprint, Poisson(4000/s) arrivals for "
                        f"{seconds:g} s: 10% 512x512 1-spp 4-bounce renders, 90% closest-hit queries of "
@@ -171,6 +184,8 @@ def bench_served(ctx, threads: int, seconds: float, fifo_seconds: float):
            "fifo": {"sample": f"first {fifo_seconds:g} s of the same arrival sequence, plain kernel one "
                               f"request at a time", **fifo},
            "ratio": round(served["mrays_per_s"] / max(fifo["mrays_per_s"], 1e-9), 3)}
+    if capacity:
+        out["capacity"] = capacity
     return out
 
 
@@ -299,6 +314,9 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--fifo-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rates", default="",
+                    help="comma-separated arrival rates (per s) for extra 6-second runs of the batching server "
+                         "(capacity beyond the 4000/s workload); off by default")
     ap.add_argument("--layout", choices=["w4", "q16", "f32"], default="w4",
                     help="node records: 4-wide 64-byte 16-bit (default), binary 32-byte 16-bit, binary 64-byte fp32")
     ap.add_argument("--refill", type=int, default=0, help="idle lanes before a wave refetches (0 = library default)")
@@ -429,7 +447,8 @@ def main():
     if not args.no_batched:
         out["batched"] = bench_batched(ctx, threads, 5)
     if not args.no_served:
-        out["served"] = bench_served(ctx, threads, args.serve_seconds, args.fifo_seconds)
+        rates = [int(x) for x in args.serve_rates.split(",") if x.strip()]
+        out["served"] = bench_served(ctx, threads, args.serve_seconds, args.fifo_seconds, rates)
 
     if args.copy_probe_mib:
         src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
