@@ -183,7 +183,8 @@ int launch_batch_kernel(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
     const uint32_t blocks = b->grid_blocks;
     if (!blocks) return RT_OK;
     int32_t* spill = RING ? b->d_spill : nullptr;
-    TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, b->d_cursor, b->total, b->cfg.refill, nullptr};
+    TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, b->d_cursor, b->total, b->cfg.refill, nullptr,
+               b->total >= (1u << 16) ? kSegments : 1u};
     BatchIo bio{b->d_table, b->d_offsets, b->d_chunk_item, b->cfg.regroup ? b->d_perm : nullptr};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), 0, s, io, bio, spill);
     RT_HIP(ctx, hipGetLastError());
@@ -211,7 +212,7 @@ int launch_batch_block(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
 // The whole launch sequence on `s` (also what the graph captures).
 int launch_sequence(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
     if (b->dry) return b->cfg.block == 128 ? launch_batch_block<128>(ctx, b, s) : launch_batch_block<256>(ctx, b, s);
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, b->d_cursor, 0u, 1u);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, b->d_cursor, 0u, kCursorBlock);
     RT_HIP(ctx, hipGetLastError());
     if (b->cfg.regroup) {
         const uint32_t nh = (uint32_t)b->items.size() * kBins;
@@ -288,8 +289,9 @@ int launch_dyn_t(rt_ctx* ctx, const BatchIo& bio, uint32_t* cursor, uint32_t tot
     uint32_t blocks = (uint32_t)ctx->num_cus * (uint32_t)per_cu;
     uint32_t need = (total + 255u) / 256u;
     if (need < blocks) blocks = need;
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, cursor, 0u, 1u);
-    TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, cursor, total, refill, nullptr};
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, cursor, 0u, kCursorBlock);
+    TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, cursor, total, refill, nullptr,
+               total >= (1u << 16) ? kSegments : 1u};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, io, bio, (int32_t*)nullptr);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;
@@ -374,7 +376,7 @@ extern "C" int rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, uint32_t
         if ((rc = up((void**)&b->d_table, table.data(), sizeof(BatchEntry) * table.size()))) break;
         if ((rc = up((void**)&b->d_offsets, offsets.data(), sizeof(uint32_t) * offsets.size()))) break;
         if ((rc = up((void**)&b->d_chunk_item, chunk_item.data(), sizeof(uint32_t) * chunk_item.size()))) break;
-        if ((rc = up((void**)&b->d_cursor, nullptr, sizeof(uint32_t)))) break;
+        if ((rc = up((void**)&b->d_cursor, nullptr, sizeof(uint32_t) * kCursorBlock))) break;
         if ((rc = up((void**)&b->d_hist, nullptr, sizeof(uint32_t) * kBins * std::max(n, 1u)))) break;
         if ((rc = up((void**)&b->d_bin_rank, nullptr, sizeof(uint32_t) * 2 * std::max(b->total, 1u)))) break;
         if ((rc = up((void**)&b->d_perm, nullptr, sizeof(uint32_t) * std::max(b->total, 1u)))) break;

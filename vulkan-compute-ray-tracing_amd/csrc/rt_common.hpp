@@ -208,7 +208,14 @@ int set_error(rt_ctx* ctx, int code, const std::string& msg);
 int obj_triangles(const char* path, uint32_t material, std::vector<rt_triangle>& tris, std::string& err);
 int hip_error(rt_ctx* ctx, hipError_t e, const char* what);
 enum { kWsRender = 0, kWsStack = 1, kWsStats = 2, kWsSpill = 3, kWsBuild = 4, kWsOrder = 5 };
-constexpr uint32_t kCursorSlots = 1024;
+// Persistent launches take their ray cursors from a ring of blocks; a block
+// holds kSegments counters kCursorStride words apart (one 128-byte line
+// each), one per segment of the ray range.
+constexpr uint32_t kSegments = 8;               // one per XCD of the MI355X
+constexpr uint32_t kCursorStride = 32;
+constexpr uint32_t kCursorBlock = kSegments * kCursorStride;
+constexpr uint32_t kCursorBlocks = 256;
+constexpr uint32_t kCursorSlots = kCursorBlocks * kCursorBlock;
 int ensure_workspace(rt_ctx* ctx, int slot, size_t bytes);
 
 #define RT_HIP(ctx, call)                                              \
@@ -244,7 +251,8 @@ int launch_sphere_pass(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
 int launch_fill_u32(rt_ctx* ctx, uint32_t* p, uint32_t value, uint32_t count, hipStream_t s);
 
 // Batched traversal from caller-filled device tables (rt_batch.hip): the
-// server's per-slot launches.  stack_need = 3 * max 4-wide height + 1.
+// server's per-slot launches.  stack_need = 3 * max 4-wide height + 1;
+// `cursor` holds kCursorBlock words (reset by the launch).
 int launch_batch_dynamic(rt_ctx* ctx, const BatchIo& bio, uint32_t* cursor, uint32_t total, uint32_t stack_need,
                          uint32_t refill, hipStream_t s);
 

@@ -147,7 +147,7 @@ int make_slot(rt_server* S, Slot& s) {
               hm((void**)&s.h_rays, (size_t)R * sizeof(rt_ray)) && hm((void**)&s.h_hits, (size_t)R * sizeof(rt_hit)) &&
               dm((void**)&s.d_table, kMaxItems * sizeof(BatchEntry)) && hm((void**)&s.h_table, kMaxItems * sizeof(BatchEntry)) &&
               dm((void**)&s.d_offsets, (kMaxItems + 1) * 4) && hm((void**)&s.h_offsets, (kMaxItems + 1) * 4) &&
-              dm((void**)&s.d_chunk, chunks * 4) && hm((void**)&s.h_chunk, chunks * 4) && dm((void**)&s.d_cursor, 4) &&
+              dm((void**)&s.d_chunk, chunks * 4) && hm((void**)&s.h_chunk, chunks * 4) && dm((void**)&s.d_cursor, 4 * kCursorBlock) &&
               dm((void**)&s.d_target, (size_t)P * 4) && dm((void**)&s.d_accum, (size_t)P * 4) &&
               hm((void**)&s.h_target, (size_t)P * 4) && hm((void**)&s.h_counters, 64 * 4);
     if (!ok) return fail(S, RT_ERR_NOMEM, "slot buffers");

@@ -118,8 +118,8 @@ constexpr int kBlock = 256;
 // different streams never share one; each is zeroed by a fill kernel on the
 // launch's own stream right before use.
 uint32_t* next_cursor(rt_ctx* ctx) {
-    uint32_t k = ctx->cursor_next++ % kCursorSlots;
-    return ctx->d_cursors + k;
+    uint32_t k = ctx->cursor_next++ % kCursorBlocks;
+    return ctx->d_cursors + (size_t)k * kCursorBlock;
 }
 
 // Trace order of a query: rays sorted by the Morton code of their origin's
@@ -201,14 +201,16 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
         spill = (int32_t*)ctx->ws[kWsSpill];
     }
     uint32_t* cursor = next_cursor(ctx);
-    int rc = launch_fill_u32(ctx, cursor, 0u, 1u, s);
+    int rc = launch_fill_u32(ctx, cursor, 0u, kCursorBlock, s);
     if (rc) return rc;
+    // segment the ray range per XCD when there is enough of it
+    const uint32_t segs = (a.n_dev || a.n >= (1u << 16)) ? kSegments : 1u;
     // 1..64 idle lanes trigger a refetch (64: only when the whole wave is idle)
     uint32_t refill = ctx->tuning.refill ? ctx->tuning.refill : 32u;
     if (refill > 64u) refill = 64u;
     TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->frame,
                   LAYOUT == kLayoutW4 ? sc->root4 : sc->root_ref};
-    TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill, a.order};
+    TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill, a.order, segs};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, ts, io, spill);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;

@@ -133,6 +133,7 @@ struct TraceIo {
     uint32_t n_host;
     uint32_t refill;
     const uint32_t* order;    // trace order (ray indices), or null: index order
+    uint32_t segs;            // segments of the ray range (1 or kSegments), see below
 };
 
 __device__ __forceinline__ float lo16(uint32_t w) { return (float)(w & 0xffffu); }
@@ -176,7 +177,16 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     const size_t gtid = (size_t)blockIdx.x * BLOCK + tid;
 
     bool busy = false;
-    bool drained = false;           // wave-uniform: the cursor passed n
+    bool drained = false;           // wave-uniform: every segment is used up
+    // Segments: the ray range (all items' rays, for batched launches) is cut into io.segs
+    // contiguous pieces, each with its own cursor (io.cursor[k * kCursorStride]).
+    // Blocks are dispatched to the 8 XCDs round robin, so block b starts on
+    // segment b % segs: each XCD's L2 then serves one region of the (sorted)
+    // rays.  A wave whose segment is used up moves to the next one, and it is
+    // drained once it has found all of them used up.  Correctness does not
+    // depend on the dispatch order, only locality does.
+    uint32_t seg = (uint32_t)(blockIdx.x % io.segs);
+    uint32_t tried = 0;
     uint32_t ray_i = 0;
     RayIn r{};
     f3 bo{0.0f, 0.0f, 0.0f};        // ray origin in the box frame
@@ -217,12 +227,18 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
         const uint32_t n_idle = (uint32_t)__popcll(idle);
         if (!drained && n_idle >= io.refill) {
             const uint32_t leader = (uint32_t)__builtin_ctzll(idle);
+            const uint32_t seg_lo = (uint32_t)((uint64_t)n * seg / io.segs);
+            const uint32_t seg_hi = (uint32_t)((uint64_t)n * (seg + 1) / io.segs);
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(io.cursor, n_idle);
+            if (lane == leader) base = atomicAdd(io.cursor + (size_t)seg * kCursorStride, n_idle);
             base = __shfl(base, (int)leader);
-            if (base + n_idle >= n) drained = true;
+            if ((uint64_t)seg_lo + base + n_idle >= seg_hi) {     // this segment is used up
+                if (++tried >= io.segs) drained = true;
+                else seg = seg + 1 == io.segs ? 0u : seg + 1;
+            }
             if (!busy) {
-                uint32_t mine = base + (uint32_t)__popcll(idle & lanes_below);
+                const uint64_t m = (uint64_t)seg_lo + base + (uint32_t)__popcll(idle & lanes_below);
+                const uint32_t mine = m < seg_hi ? (uint32_t)m : n;
                 if (mine < n) {
                     QFrame q = sc.frame;
                     int32_t root = sc.root;
