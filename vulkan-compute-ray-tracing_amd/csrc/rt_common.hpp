@@ -211,7 +211,8 @@ enum { kWsRender = 0, kWsStack = 1, kWsStats = 2, kWsSpill = 3, kWsBuild = 4, kW
 // Persistent launches take their ray cursors from a ring of blocks; a block
 // holds kSegments counters kCursorStride words apart (one 128-byte line
 // each), one per segment of the ray range.
-constexpr uint32_t kSegments = 8;               // one per XCD of the MI355X
+constexpr uint32_t kXcds = 8;                   // XCDs of the MI355X
+constexpr uint32_t kSegments = 8;               // one segment per XCD (32, four per XCD, measured no better)
 constexpr uint32_t kCursorStride = 32;
 constexpr uint32_t kCursorBlock = kSegments * kCursorStride;
 constexpr uint32_t kCursorBlocks = 256;

@@ -180,12 +180,16 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     bool drained = false;           // wave-uniform: every segment is used up
     // Segments: the ray range (all items' rays, for batched launches) is cut into io.segs
     // contiguous pieces, each with its own cursor (io.cursor[k * kCursorStride]).
-    // Blocks are dispatched to the 8 XCDs round robin, so block b starts on
-    // segment b % segs: each XCD's L2 then serves one region of the (sorted)
-    // rays.  A wave whose segment is used up moves to the next one, and it is
+    // Blocks are dispatched to the 8 XCDs round robin, so each XCD's blocks
+    // start on one run of adjacent segments: each XCD's L2 then serves one
+    // region of the (sorted) rays, and the cursors' atomics are spread out.  A wave whose segment is used up moves to the next one, and it is
     // drained once it has found all of them used up.  Correctness does not
     // depend on the dispatch order, only locality does.
-    uint32_t seg = (uint32_t)(blockIdx.x % io.segs);
+    // block b runs on XCD b % kXcds; with kSegments segments each XCD starts
+    // on its own run of kSegments / kXcds adjacent ones
+    uint32_t seg = io.segs == kSegments
+                       ? (blockIdx.x % kXcds) * (kSegments / kXcds) + (blockIdx.x / kXcds) % (kSegments / kXcds)
+                       : (uint32_t)(blockIdx.x % io.segs);
     uint32_t tried = 0;
     uint32_t ray_i = 0;
     RayIn r{};
