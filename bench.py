@@ -189,24 +189,76 @@ def bench_served(ctx, threads: int, seconds: float, fifo_seconds: float, rates=(
     return out
 
 
+def lavapipe_probe() -> str:
+    """Whether the reference shader could run on Mesa's CPU Vulkan driver here."""
+    import ctypes.util
+    import glob
+    import shutil
+    missing = []
+    if not ctypes.util.find_library("vulkan"):
+        missing.append("libvulkan.so.1")
+    icds = [f for d in ("/usr/share/vulkan/icd.d", "/etc/vulkan/icd.d", "/usr/local/share/vulkan/icd.d")
+            for f in glob.glob(f"{d}/lvp_icd*.json")]
+    if not icds:
+        missing.append("lvp_icd*.json")
+    if not (shutil.which("glslangValidator") or shutil.which("glslc")):
+        missing.append("glslangValidator/glslc")
+    if missing:
+        return "absent (missing: " + ", ".join(missing) + ")"
+    return "present (not used: no Vulkan host in this build)"
+
+
 def cpu_baseline(hs, rays_np, seconds: float, threads: int):
+    """The oracle (CPU restatement of the shader) on the host cores: the
+    4M-triangle query on a calibrated sample (one warm-up, median of 5), and
+    beside it the small render and the icosphere query."""
+    import numpy as np
     import oracle as O
+    import rtamd
     osc = O.Scene(hs.triangles, hs.materials, hs.nodes, hs.lights, hs.spheres)
+    per_run = seconds / 5.0
     m = 64 * threads
-    for _ in range(6):
+    for _ in range(8):                          # calibration; the last run is the warm-up
         t0 = time.perf_counter()
         O.closest_hit(osc, rays_np[:m], O.TRAV_REFERENCE, threads=threads)
         dt = time.perf_counter() - t0
-        if dt >= 0.5 * seconds or m == len(rays_np):
+        if dt >= 0.6 * per_run or m == len(rays_np):
             break
-        m = int(min(len(rays_np), max(2 * m, m * seconds / max(dt, 1e-3))))
+        m = int(min(len(rays_np), max(2 * m, m * per_run / max(dt, 1e-3))))
+    runs = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.closest_hit(osc, rays_np[:m], O.TRAV_REFERENCE, threads=threads)
+        runs.append(time.perf_counter() - t0)
+    dt = sorted(runs)[2]
     k = min(CANON_SAMPLE, len(rays_np))
     _, nv, tt, _ = O.closest_hit(osc, rays_np[:k], O.TRAV_ORDERED, stats=True, threads=threads)
+
+    # the reference's small render and the icosphere query, same inputs as the GPU side
+    soup = rtamd.scene_soup(1024, 1, threads=threads)
+    t0 = time.perf_counter()
+    _, small_rays = O.render(O.Scene(soup.triangles, soup.materials, soup.nodes, soup.lights, soup.spheres),
+                             [0.0, 3.0, 0.0], 256, 256, vfov=60.0, bounces=4, threads=threads)
+    small_s = time.perf_counter() - t0
+    ico = rtamd.scene_icosphere(6, 2, 0.1, 4.0, threads=threads)
+    isc = O.Scene(ico.triangles, ico.materials, ico.nodes, ico.lights, ico.spheres)
+    coh = rtamd.rays_camera_tiles([0, 3, 0], 60.0, 1024, 1024, 8, 0, 524288)[::8]   # spread over the half image
+    inc = rtamd.rays_shell(65536, 2024, 3.0, 0.5)
+    medium = {}
+    for name, rr in (("coherent", coh), ("incoherent", inc)):
+        t0 = time.perf_counter()
+        O.closest_hit(isc, rr, O.TRAV_REFERENCE, threads=threads)
+        medium[f"{name}_mrays_per_s"] = round(len(rr) / (time.perf_counter() - t0) / 1e6, 4)
     return {"value": round(m / dt / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"first {m} of the {len(rays_np)} incoherent rays, oracle hit_bvh in the shader's "
-                      f"visiting order (line test, both children pushed), {threads} threads, {dt:.1f} s; "
-                      "CPU restatement, not lavapipe",
-            "cpu_model": cpu_model(),
+                      f"visiting order (line test, both children pushed), {threads} threads, median of 5 runs "
+                      f"of {dt:.1f} s after a warm-up; CPU restatement, not lavapipe",
+            "cpu_model": cpu_model(), "lavapipe": lavapipe_probe(),
+            "small_render": {"rays": int(small_rays), "s": round(small_s, 3),
+                             "mrays_per_s": round(small_rays / small_s / 1e6, 4),
+                             "sample": "the whole 256x256, 4-bounce render of the 1024-triangle soup"},
+            "query_medium": dict(medium, sample="every 8th of the 524288 coherent camera rays and the first "
+                                                "65536 incoherent shell rays on the level-6 icosphere"),
             "canonical": {"rays": k, "traversal": "oracle ordered, t-culled (nearer child first)",
                           "node_visits_per_ray": round(float(nv.mean()), 3),
                           "tri_tests_per_ray": round(float(tt.mean()), 3)}}
