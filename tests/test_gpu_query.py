@@ -286,3 +286,26 @@ def test_query_stage_timing(rt, gpu_ctx):
         gpu_ctx.set_tuning()
     gpu_ctx.closest_hit(scene, rays)
     assert gpu_ctx.last_query_stats().timed == 0
+
+
+@pytest.mark.parametrize("n", [65535, 65536, 65537, 100003, 262151])
+def test_every_ray_traced_once_at_odd_sizes(rt, gpu_ctx, n):
+    """Segmented cursors and sorted order: at sizes around the thresholds and
+    not divisible by the segment count, every ray gets its own hit (the
+    buffer starts as a sentinel) and the hits equal the shader-order ones."""
+    hs = rt.scene_clusters(1 << 16, 25)
+    scene = gpu_ctx.upload(hs)
+    rays_np = rt.rays_box(n, 26 + n % 7, 4.0)
+    rays = rt.rays_to_device(rays_np)
+    print(f"case: {n} rays, sentinel-filled hits", flush=True)
+    for mode in (rt.ORDER_SORTED, rt.ORDER_GIVEN):
+        gpu_ctx.set_tuning(ray_order=mode)
+        try:
+            hits = torch.full((n, 4), float("nan"), device="cuda")
+            gpu_ctx.closest_hit(scene, rays, hits)
+            got = rt.hits_from_device(hits)
+        finally:
+            gpu_ctx.set_tuning()
+        assert not np.isnan(got["t"]).any()
+        ref = rt.hits_from_device(gpu_ctx.closest_hit(scene, rays, traversal=rt.TRAVERSE_REFERENCE))
+        assert_equal_up_to_ties(oscene(hs), rays_np, got, ref)
