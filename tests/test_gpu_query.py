@@ -309,3 +309,20 @@ def test_every_ray_traced_once_at_odd_sizes(rt, gpu_ctx, n):
         assert not np.isnan(got["t"]).any()
         ref = rt.hits_from_device(gpu_ctx.closest_hit(scene, rays, traversal=rt.TRAVERSE_REFERENCE))
         assert_equal_up_to_ties(oscene(hs), rays_np, got, ref)
+
+
+def test_bench_configuration_against_the_oracle(rt, gpu_ctx):
+    """The exact bench workload -- 4,194,304-triangle clustered soup (seed 3),
+    4,194,304 incoherent rays (seed 4) -- through the default path (sorted
+    trace order, XCD segments, 4-wide nodes), checked against the oracle's
+    ordered traversal on every 64th ray: ids equal up to near-ties, t/u/v
+    bit-equal where they agree."""
+    hs = rt.scene_clusters(1 << 22, 3)
+    scene = gpu_ctx.upload(hs)
+    rays_np = rt.rays_box(1 << 22, 4, 4.0)
+    print("case: bench scene, 4M rays, oracle on every 64th", flush=True)
+    got = rt.hits_from_device(gpu_ctx.closest_hit(scene, rt.rays_to_device(rays_np)))
+    sample = rays_np[::64]
+    want = O.closest_hit(oscene(hs), sample, O.TRAV_ORDERED, threads=16)
+    assert (got["id"][::64] >= 0).mean() > 0.2
+    assert assert_equal_up_to_ties(oscene(hs), sample, got[::64], want) <= 4

@@ -129,3 +129,17 @@ def test_stage_timing_leaves_the_image_unchanged(rt, gpu_ctx, soup):
     assert st.primary_rays == 160 * 128 and st.secondary_rays > 0
     assert st.trace_primary_ms > 0 and st.trace_secondary_ms > 0 and st.shade_ms >= 0
     assert st.trace_primary_ms + st.trace_secondary_ms <= st.total_ms * (1 + 1e-5)
+
+
+def test_render_with_hit_scene_sphere_rule(rt, gpu_ctx, soup):
+    """RT_SPHERES_SCENE in a render: a sphere hit overrides a nearer triangle
+    (hit_scene, ray-trace-compute.comp:222-247); shader-order traversal,
+    bit-exact against the oracle."""
+    hs, scene = soup
+    print("case: render with hit_scene's sphere rule", flush=True)
+    img = gpu_ctx.render(scene, CAM, 128, 128, vfov=60.0, bounces=4, spheres=2,
+                         traversal=rt.TRAVERSE_REFERENCE).cpu().numpy()
+    want, _ = O.render(oscene(hs), CAM, 128, 128, vfov=60.0, bounces=4, spheres=2, threads=8)
+    assert np.array_equal(img, want)
+    near, _ = O.render(oscene(hs), CAM, 128, 128, vfov=60.0, bounces=4, spheres=1, threads=8)
+    assert not np.array_equal(want, near)            # the two rules differ on this scene
