@@ -143,3 +143,21 @@ def test_render_with_hit_scene_sphere_rule(rt, gpu_ctx, soup):
     assert np.array_equal(img, want)
     near, _ = O.render(oscene(hs), CAM, 128, 128, vfov=60.0, bounces=4, spheres=1, threads=8)
     assert not np.array_equal(want, near)            # the two rules differ on this scene
+
+
+def test_bench_render_against_the_oracle_on_sampled_rows(rt, gpu_ctx):
+    """The bench render -- 4,194,304-triangle clustered soup, 1920x1080,
+    4 progressive frames, 8 bounces, vfov 60, eye (0,0,10) -- through the
+    default path (sorted secondary bounces, XCD segments), against the
+    oracle's ordered traversal on every 72nd pixel row: RMSE <= 1e-3."""
+    hs = rt.scene_clusters(1 << 22, 3)
+    scene = gpu_ctx.upload(hs)
+    print("case: bench render 1920x1080x4, 8 bounces, oracle on every 72nd row", flush=True)
+    tgt, _ = gpu_ctx.render_frames(scene, [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8)
+    torch.cuda.synchronize()
+    got = tgt.cpu().numpy()[::72]
+    want, _ = O.render_frames(oscene(hs), [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8,
+                              traversal=O.TRAV_ORDERED, row_step=72, threads=16)
+    want = want[::72]
+    assert got[..., :3].mean() > 1.0
+    assert rmse(got, want) <= 1e-3
