@@ -1,4 +1,6 @@
-"""Builds librtamd.so (HIP for gfx950 + host C++) in-tree.
+"""Builds librtamd.so (HIP for gfx950 + host C++) in-tree, and the C++
+host program rtamd_frame_loop (host/frame_loop_main.cpp over rtamd.hpp)
+linked against it.
 
     python vulkan-compute-ray-tracing_amd/build.py [--jobs N]
 
@@ -20,6 +22,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "librtamd.so")
+HOST_MAIN = os.path.join(PKG, "host", "frame_loop_main.cpp")
+HOST_BIN = os.path.join(PKG, "rtamd_frame_loop")
 ARCH = os.environ.get("RTAMD_ARCH", "gfx950")
 
 SOURCES = [
@@ -75,9 +79,26 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_host_program()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {HOST_BIN}")
     return LIB
+
+
+def build_host_program() -> str:
+    """g++ over the C-ABI headers only: no HIP header or flag is needed by a
+    host program that uses the library."""
+    gxx = shutil.which("g++") or shutil.which("c++")
+    if not gxx:
+        raise RuntimeError("g++ not found: the host program needs a C++17 compiler")
+    tmp = HOST_BIN + ".tmp"
+    cmd = [gxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+           HOST_MAIN, "-o", tmp, "-L", PKG, "-lrtamd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host program build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, HOST_BIN)
+    return HOST_BIN
 
 
 if __name__ == "__main__":
