@@ -19,6 +19,10 @@ sample of the same rays), and side measurements:
   render   the 1920x1080 x 4 spp, 8-bounce render of the same scene;
   device_build  the GPU linear-BVH builder on the same triangles, the host
            builder's time beside it, and the query on the device-built tree;
+  small_render, query_medium  the 1,024-triangle soup render and the
+           81,920-triangle icosphere query;
+  doge_frame_loop  the reference application's own frame: its doge scene at
+           800x600, two bounces, render + accumulation copy + display pass;
   batched  64 clustered-soup scenes (triangles log-uniform in [1k, 1M], seeds
            100+i) with ray batches log-uniform in [4k, 2M] (seeds 200+i):
            64 plain launches against one autotuned persistent launch
@@ -346,6 +350,57 @@ def bench_small_and_medium(ctx, threads: int):
                            "coherent_ms": round(t_coh, 4), "coherent_mrays_per_s": round(524288 / t_coh / 1e3, 1),
                            "incoherent_ms": round(t_inc, 4),
                            "incoherent_mrays_per_s": round(524288 / t_inc / 1e3, 1)}
+    out["doge_frame_loop"] = bench_doge_frame_loop(ctx)
+    return out
+
+
+DOGE_TRIANGLES = os.path.join(ROOT, "tests", "golden", "doge_triangles.npy")
+
+
+def bench_doge_frame_loop(ctx, frames: int = 60):
+    """The reference application's own workload: its doge scene (1,422
+    triangles, RtScene.h:44-101) in an 800x600 window with the shader's vfov 30
+    and two bounces, camera of main.cpp:37, one compute pass plus the display
+    pass per frame -- the frame whose time main.cpp:406-412 prints."""
+    import numpy as np
+    import torch
+    import rtamd
+    tris = np.frombuffer(bytearray(np.load(DOGE_TRIANGLES).tobytes()), rtamd.TRI_DT).copy()
+    mats = np.zeros(6, rtamd.MAT_DT)                                     # RtScene.h:49-62
+    mats["type"] = [rtamd.RT_MAT_LAMBERTIAN] * 3 + [rtamd.RT_MAT_LIGHT, rtamd.RT_MAT_METAL, rtamd.RT_MAT_GLASS]
+    mats["albedo"] = np.array([[.3, .3, .3], [.9, .1, .1], [.1, .9, .1], [2, 2, 2], [1, 1, 1], [1, 1, 1]], np.float32)
+    spheres = np.zeros(1, rtamd.SPHERE_DT)                               # RtScene.h:97
+    spheres["s"] = np.array([[0.6, 1.0, -1.0, 0.6]], np.float32)
+    spheres["materialIndex"] = 5
+    hs = rtamd.HostScene(tris, mats, rtamd.bvh_build(tris, seed=1), rtamd.scene_lights(tris, mats), spheres)
+    sc = ctx.upload(hs)
+    cam = [1.8, 8.6, 1.1]
+    loop = rtamd.FrameLoop(ctx, sc, 800, 600, vfov=30.0, bounces=2)
+    try:
+        for k in range(5):
+            loop.step(cam, time=0.0)
+            loop.display()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        for k in range(frames):
+            loop.step(cam, time=0.0)
+            loop.display()
+        e1.record()
+        torch.cuda.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / frames
+        gpu_ms = e0.elapsed_time(e1) / frames
+        prim, sec = ctx.last_rays()
+    finally:
+        loop.close()
+        sc.release()
+    out = {"workload": "doge scene (1422 triangles), 800x600, 1 spp per frame, 2 bounces, vfov 30, camera "
+                       "(1.8, 8.6, 1.1); per frame: render, target->accumulation copy, display pass",
+           "frames": frames, "ms_per_frame": round(wall_ms, 4), "gpu_ms_per_frame": round(gpu_ms, 4),
+           "frames_per_s": round(1e3 / wall_ms, 1), "rays_per_frame": prim + sec,
+           "mrays_per_s": round((prim + sec) / wall_ms / 1e3, 1)}
+    log(f"doge frame loop: {wall_ms:.3f} ms/frame ({out['frames_per_s']} frames/s)")
     return out
 
 

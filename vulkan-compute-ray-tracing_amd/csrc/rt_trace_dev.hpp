@@ -293,7 +293,14 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                 const uint4* np = node ? reinterpret_cast<const uint4*>(l_w4 + cur)
                                        : reinterpret_cast<const uint4*>(l_tris + ~cur);
                 const uint4 a = np[0], b = np[1], c = np[2];
-                const uint4 dw = np[node ? 3 : 0];
+                uint4 dw = np[node ? 3 : 0];
+                // Only the node branch reads dw, so the compiler would sink its
+                // load into that branch, after the triangle branch has already
+                // waited for the other three: two round trips for a wave that
+                // holds both.  This empty statement uses dw here, before the
+                // branch, so the four loads are issued together and waited for
+                // once.
+                asm volatile("" : "+v"(dw.x), "+v"(dw.y), "+v"(dw.z), "+v"(dw.w));
                 if (node) {
                     const int4 d = make_int4((int)dw.x, (int)dw.y, (int)dw.z, (int)dw.w);
                     if (STATS) nv++;
