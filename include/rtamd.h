@@ -202,7 +202,9 @@ enum { RT_ORDER_AUTO = 0, RT_ORDER_GIVEN = 1, RT_ORDER_SORTED = 2 };
 typedef struct rt_tuning {
     uint32_t node_layout;   /* RT_LAYOUT_*: binary 64-byte fp32 or 32-byte 16-bit
                                nodes, or 4-wide 64-byte 16-bit nodes            */
-    uint32_t refill;        /* idle lanes (1..64) that make a wave fetch rays     */
+    uint32_t refill;        /* idle lanes (1..64) that make a wave fetch rays;
+                               0 = by scene: 16 from 2^17 triangles up (long
+                               traversals), 48 below (short ones)                */
     uint32_t blocks_per_cu; /* persistent blocks per CU (0 = occupancy limit)     */
     uint32_t stack_lds;     /* 4-wide layout: 0 = whole traversal stack in LDS;
                                8 or 16 = that many top entries in LDS, deeper ones

@@ -213,6 +213,8 @@ enum { kWsRender = 0, kWsStack = 1, kWsStats = 2, kWsSpill = 3, kWsBuild = 4, kW
 // each), one per segment of the ray range.
 constexpr uint32_t kXcds = 8;                   // XCDs of the MI355X
 constexpr uint32_t kSegments = 8;               // one segment per XCD (32, four per XCD, measured no better)
+constexpr uint32_t kSegmentShift = 3;           // log2(kSegments)
+static_assert(kSegments == 1u << kSegmentShift, "segments are a power of two");
 constexpr uint32_t kCursorStride = 32;
 constexpr uint32_t kCursorBlock = kSegments * kCursorStride;
 constexpr uint32_t kCursorBlocks = 256;

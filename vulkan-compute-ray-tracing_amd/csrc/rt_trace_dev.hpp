@@ -191,6 +191,11 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                        ? (blockIdx.x % kXcds) * (kSegments / kXcds) + (blockIdx.x / kXcds) % (kSegments / kXcds)
                        : (uint32_t)(blockIdx.x % io.segs);
     uint32_t tried = 0;
+    // the segment's ray range [seg_lo, seg_hi); io.segs is 1 or kSegments,
+    // both powers of two, so the bounds are a product and a shift
+    const uint32_t seg_shift = io.segs == kSegments ? kSegmentShift : 0u;
+    auto seg_bound = [&](uint32_t k) -> uint32_t { return (uint32_t)(((uint64_t)n * k) >> seg_shift); };
+    uint32_t seg_lo = seg_bound(seg), seg_hi = seg_bound(seg + 1);
     uint32_t ray_i = 0;
     RayIn r{};
     f3 bo{0.0f, 0.0f, 0.0f};        // ray origin in the box frame
@@ -231,18 +236,22 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
         const uint32_t n_idle = (uint32_t)__popcll(idle);
         if (!drained && n_idle >= io.refill) {
             const uint32_t leader = (uint32_t)__builtin_ctzll(idle);
-            const uint32_t seg_lo = (uint32_t)((uint64_t)n * seg / io.segs);
-            const uint32_t seg_hi = (uint32_t)((uint64_t)n * (seg + 1) / io.segs);
+            const uint32_t lo = seg_lo, hi = seg_hi;
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(io.cursor + (size_t)seg * kCursorStride, n_idle);
             base = __shfl(base, (int)leader);
-            if ((uint64_t)seg_lo + base + n_idle >= seg_hi) {     // this segment is used up
-                if (++tried >= io.segs) drained = true;
-                else seg = seg + 1 == io.segs ? 0u : seg + 1;
+            if ((uint64_t)lo + base + n_idle >= hi) {     // this segment is used up
+                if (++tried >= io.segs) {
+                    drained = true;
+                } else {
+                    seg = seg + 1 == io.segs ? 0u : seg + 1;
+                    seg_lo = seg_bound(seg);
+                    seg_hi = seg_bound(seg + 1);
+                }
             }
             if (!busy) {
-                const uint64_t m = (uint64_t)seg_lo + base + (uint32_t)__popcll(idle & lanes_below);
-                const uint32_t mine = m < seg_hi ? (uint32_t)m : n;
+                const uint64_t m = (uint64_t)lo + base + (uint32_t)__popcll(idle & lanes_below);
+                const uint32_t mine = m < hi ? (uint32_t)m : n;
                 if (mine < n) {
                     QFrame q = sc.frame;
                     int32_t root = sc.root;
