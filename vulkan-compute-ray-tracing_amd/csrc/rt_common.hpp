@@ -63,8 +63,10 @@ static_assert(sizeof(PNodeQ) == 32, "quantized node record");
 // PNode4 (64 bytes, four 16-byte loads): the binary tree collapsed two
 // levels at a time -- the children of a node's children, up to four, with
 // 16-bit boxes in the same fixed-point frame as PNodeQ.  Child k's box takes
-// w[3k..3k+2] in the PNodeQ packing; ref[k] as in PNode, kNoChild for an
-// unused slot.  One record (one 64-byte line) decides two tree levels.
+// w[3k..3k+2], one word per axis with that axis' two planes:
+//   w[3k] = lo.x | hi.x << 16   w[3k+1] = lo.y | hi.y << 16   w[3k+2] = lo.z | hi.z << 16
+// ref[k] as in PNode, kNoChild for an unused slot.  One record (one 64-byte
+// line) decides two tree levels.
 struct alignas(16) PNode4 { uint32_t w[12]; int32_t ref[4]; };
 static_assert(sizeof(PNode4) == 64, "4-wide node record");
 constexpr int32_t kNoChild = INT32_MIN + 1;

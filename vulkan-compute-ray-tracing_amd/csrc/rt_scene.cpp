@@ -232,9 +232,16 @@ void widen_nodes(PackedTree& out) {
         PNode4& w = out.w4nodes[me];
         for (int k = 0; k < 4; ++k) {
             if (k < n) {
-                w.w[3 * k] = items[k].w0;
-                w.w[3 * k + 1] = items[k].w1;
-                w.w[3 * k + 2] = items[k].w2;
+                // the binary record's words hold (lo x, lo y), (lo z, hi x),
+                // (hi y, hi z); the 4-wide one pairs each axis' planes,
+                // (lo x, hi x), (lo y, hi y), (lo z, hi z), so one rotate by
+                // 16 bits puts the plane a ray enters first in the low half
+                const uint32_t lx = items[k].w0 & 0xffffu, ly = items[k].w0 >> 16;
+                const uint32_t lz = items[k].w1 & 0xffffu, hx = items[k].w1 >> 16;
+                const uint32_t hy = items[k].w2 & 0xffffu, hz = items[k].w2 >> 16;
+                w.w[3 * k] = lx | hx << 16;
+                w.w[3 * k + 1] = ly | hy << 16;
+                w.w[3 * k + 2] = lz | hz << 16;
                 w.ref[k] = items[k].ref;     // patched below for internal children
             } else {
                 w.w[3 * k] = w.w[3 * k + 1] = w.w[3 * k + 2] = 0;

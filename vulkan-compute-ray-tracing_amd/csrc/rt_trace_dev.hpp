@@ -44,9 +44,18 @@ __device__ __forceinline__ void store_hit_stream(rt_hit* hits, uint32_t i, int32
     __builtin_nontemporal_store(h, reinterpret_cast<v4f*>(hits) + i);
 }
 
-// Conservative slab test of one box against [tmin, tmax].  Rounding of the
-// reciprocal form is absorbed by a relative slack of 2^-16 (far larger than
-// the few-ulp error), so a box that holds an accepted hit is never culled.
+// Accept test of a slab interval: near - |near| * 2^-15 <= far, one fused
+// multiply-add and a compare.  The computed near and far each carry a few ulp
+// (< 2^-21 relative) of rounding; where the exact interval is non-empty and
+// rounding inverted it, near and far are within that error of each other, so
+// 2^-15 |near| covers both.  Where |far| is much larger than |near| and the
+// exact interval is non-empty, far is positive and far above near already.
+__device__ __forceinline__ bool slab_accept(float nearv, float farv) {
+    return __builtin_fmaf(-fabsf(nearv), 0x1p-15f, nearv) <= farv;
+}
+
+// Conservative slab test of one box against [tmin, tmax], so a box that
+// holds an accepted hit is never culled (slab_accept).
 __device__ __forceinline__ bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz,
                                      const f3& o, const f3& inv, float tmin, float tmax, float& tnear) {
     float tx0 = (lox - o.x) * inv.x, tx1 = (hix - o.x) * inv.x;
@@ -55,7 +64,23 @@ __device__ __forceinline__ bool slab(float lox, float loy, float loz, float hix,
     float nearv = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
     float farv = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
     tnear = nearv;
-    return nearv <= farv + (fabsf(farv) + fabsf(nearv)) * 0x1p-16f;
+    return slab_accept(nearv, farv);
+}
+
+// The same test with each axis' planes already in the order the ray meets
+// them: (nx, ny, nz) the planes it enters by, (fx, fy, fz) those it leaves
+// by.  For inv >= 0 that is the low plane first, for inv < 0 the high one;
+// rounding is monotonic, so these t are exactly slab()'s min and max and the
+// two tests decide alike, with four min/max fewer per box.
+__device__ __forceinline__ bool slab_ordered(float nx, float ny, float nz, float fx, float fy, float fz,
+                                             const f3& o, const f3& inv, float tmin, float tmax, float& tnear) {
+    const float tnx = (nx - o.x) * inv.x, tfx = (fx - o.x) * inv.x;
+    const float tny = (ny - o.y) * inv.y, tfy = (fy - o.y) * inv.y;
+    const float tnz = (nz - o.z) * inv.z, tfz = (fz - o.z) * inv.z;
+    const float nearv = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+    const float farv = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+    tnear = nearv;
+    return slab_accept(nearv, farv);
 }
 
 // triIntersect + hit_triangle's accept test on a packed triangle.
@@ -200,6 +225,9 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     RayIn r{};
     f3 bo{0.0f, 0.0f, 0.0f};        // ray origin in the box frame
     f3 inv{0.0f, 0.0f, 0.0f};       // reciprocal direction in the box frame
+    // 4-wide records: 16 where the ray runs toward -axis (its entry plane is
+    // then the word's high half), so a rotate puts the entry plane low
+    uint32_t rot_x = 0, rot_y = 0, rot_z = 0;
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
     int32_t best = -1;
     uint32_t nv = 0, nt = 0;
@@ -277,6 +305,9 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                         bo = f3{(r.o.x - q.ox) * q.rx, (r.o.y - q.oy) * q.ry, (r.o.z - q.oz) * q.rz};
                         inv = f3{inv.x * q.sx, inv.y * q.sy, inv.z * q.sz};
                     }
+                    rot_x = (__float_as_uint(inv.x) >> 27) & 16u;
+                    rot_y = (__float_as_uint(inv.y) >> 27) & 16u;
+                    rot_z = (__float_as_uint(inv.z) >> 27) & 16u;
                     best_t = r.tmax; bu = 0.0f; bv = 0.0f; best = -1;
                     nv = 0; nt = 0; sp = 0;
                     cur = root;
@@ -316,14 +347,21 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     float k0 = __builtin_inff(), k1 = __builtin_inff(), k2 = __builtin_inff(), k3 = __builtin_inff();
                     float tn;
                     int nh = 0;
-                    if (d.x != kNoChild && slab(lo16(a.x), hi16(a.x), lo16(a.y), hi16(a.y), lo16(a.z), hi16(a.z), bo, inv,
-                                                r.tmin, best_t, tn)) { k0 = tn; nh++; }
-                    if (d.y != kNoChild && slab(lo16(a.w), hi16(a.w), lo16(b.x), hi16(b.x), lo16(b.y), hi16(b.y), bo, inv,
-                                                r.tmin, best_t, tn)) { k1 = tn; nh++; }
-                    if (d.z != kNoChild && slab(lo16(b.z), hi16(b.z), lo16(b.w), hi16(b.w), lo16(c.x), hi16(c.x), bo, inv,
-                                                r.tmin, best_t, tn)) { k2 = tn; nh++; }
-                    if (d.w != kNoChild && slab(lo16(c.y), hi16(c.y), lo16(c.z), hi16(c.z), lo16(c.w), hi16(c.w), bo, inv,
-                                                r.tmin, best_t, tn)) { k3 = tn; nh++; }
+                    // child k's words, each rotated so the entry plane is the low half
+                    auto box = [&](uint32_t wx, uint32_t wy, uint32_t wz, float& key) {
+                        const uint32_t x = __builtin_amdgcn_alignbit(wx, wx, rot_x);
+                        const uint32_t y = __builtin_amdgcn_alignbit(wy, wy, rot_y);
+                        const uint32_t z = __builtin_amdgcn_alignbit(wz, wz, rot_z);
+                        if (slab_ordered(lo16(x), lo16(y), lo16(z), hi16(x), hi16(y), hi16(z), bo, inv, r.tmin,
+                                         best_t, tn)) {
+                            key = tn;
+                            nh++;
+                        }
+                    };
+                    if (d.x != kNoChild) box(a.x, a.y, a.z, k0);
+                    if (d.y != kNoChild) box(a.w, b.x, b.y, k1);
+                    if (d.z != kNoChild) box(b.z, b.w, c.x, k2);
+                    if (d.w != kNoChild) box(c.y, c.z, c.w, k3);
                     if (nh > 0) {
                         int32_t v0 = d.x, v1 = d.y, v2 = d.z, v3 = d.w;
                         // sort the four (entry distance, child) pairs; misses carry +inf
