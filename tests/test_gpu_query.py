@@ -161,6 +161,45 @@ def test_edge_rays(rt, gpu_ctx, soup):
     assert_equal_up_to_ties(oscene(hs), r, run(gpu_ctx, scene, r), ref)
 
 
+@pytest.mark.parametrize("layout", ["w4", "q16"])
+def test_far_origins(rt, gpu_ctx, soup, layout):
+    """Origins far outside the scene in units of the 16-bit frame's quantum
+    (the soup spans ~2.2 units, a quantum ~3.4e-5): near the distance where
+    the kernel starts moving the origin along the ray (2^21 quanta, ~70
+    units), far past it (1e2 .. 1e5 units, up to ~3e9 quanta), rays aimed at
+    the scene from there, rays aimed away, and rays whose tmax ends short of
+    it.  The packed kernels must give the oracle's hits (exact float boxes)."""
+    from rtamd import RAY_DT
+    hs, scene = soup
+    rng = np.random.default_rng(17)
+    parts = []
+    for dist, n in ((60.0, 4096), (75.0, 4096), (1e2, 4096), (1e3, 4096), (1e5, 4096)):
+        r = np.zeros(n, RAY_DT)
+        u = rng.normal(size=(n, 3))
+        o = u / np.linalg.norm(u, axis=1, keepdims=True) * dist
+        target = rng.uniform(-1.1, 1.1, (n, 3))
+        d = target - o
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        d[: n // 8] *= -1.0                                    # aimed away: all miss
+        r["o"], r["d"] = o, d
+        r["tmin"] = 0.001
+        r["tmax"] = np.where(np.arange(n) % 9 == 0, 0.5 * dist, 10.0 * dist)   # every 9th ends short
+        parts.append(r)
+    r = np.concatenate(parts)
+    print(f"case: far origins, {layout}, {len(r)} rays", flush=True)
+    gpu_ctx.set_tuning(node_layout=rt.LAYOUT_W4 if layout == "w4" else rt.LAYOUT_Q16)
+    try:
+        got = run(gpu_ctx, scene, r)
+    finally:
+        gpu_ctx.set_tuning()
+    want = O.closest_hit(oscene(hs), r, O.TRAV_REFERENCE, threads=8)
+    assert (want["id"] >= 0).mean() > 0.3
+    assert assert_equal_up_to_ties(oscene(hs), r, got, want) <= 4
+    local = np.arange(len(r)) % 4096
+    assert (got["id"][local < 512] == -1).all()
+    assert (got["id"][local % 9 == 0] == -1).all()
+
+
 def test_single_triangle_and_empty_batches(rt, gpu_ctx):
     hs = rt.scene_soup(1, 3)
     scene = gpu_ctx.upload(hs)

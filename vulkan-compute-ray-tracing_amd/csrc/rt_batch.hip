@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_regroup_bin(BatchIo bio, uint32_t total
         const BatchEntry& e = bio.table[b];
         const uint32_t local = g - e.base;
         const float4 o = reinterpret_cast<const float4*>(e.rays)[2 * (size_t)local];
-        // (v - org) * rs spans [0, 65000] over the item's scene; 16 cells per axis
+        // (v - org) * rs spans [2, 65002] over the item's scene; 16 cells per axis
         auto cell = [](float v, float org, float rs) -> uint32_t {
             const float q = (v - org) * rs * (16.0f / 65000.0f);
             return q >= 15.0f ? 15u : (q > 0.0f ? (uint32_t)q : 0u);

@@ -266,13 +266,18 @@ void quantize_nodes(PackedTree& out) {
         }
     }
     if (out.nodes.empty()) for (int k = 0; k < 3; ++k) lo[k] = hi[k] = 0.0;
+    // The frame puts the scene's lowest bound 2 quanta above the origin and
+    // its highest at most 65002 quanta above, so every quantized bound
+    // (rounded outward by one extra quantum below) lies strictly inside
+    // [0, 65535] with at least one full quantum between it and the exact one,
+    // also at the scene's edges.
     float org[3], scl[3];
     for (int k = 0; k < 3; ++k) {
-        org[k] = (float)lo[k];
-        if ((double)org[k] > lo[k]) org[k] = std::nextafter(org[k], -INFINITY);
-        double ext = std::max(hi[k] - (double)org[k], 1e-30);
+        const double ext = std::max(hi[k] - lo[k], 1e-30);
         scl[k] = (float)(ext / 65000.0);
         if ((double)scl[k] * 65000.0 < ext) scl[k] = std::nextafter(scl[k], INFINITY);
+        org[k] = (float)(lo[k] - 2.0 * (double)scl[k]);
+        if ((double)org[k] > lo[k] - 2.0 * (double)scl[k]) org[k] = std::nextafter(org[k], -INFINITY);
     }
     out.frame = QFrame{org[0], org[1], org[2], scl[0], scl[1], scl[2], 1.0f / scl[0], 1.0f / scl[1], 1.0f / scl[2]};
     // q covers [org + q*scl] exactly (double arithmetic of the float frame)

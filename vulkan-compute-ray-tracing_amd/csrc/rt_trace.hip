@@ -141,7 +141,7 @@ __device__ __forceinline__ uint32_t spread10(uint32_t x) {   // bit b -> bit 3b,
 }
 
 __device__ __forceinline__ uint32_t order_cell(float o, float org, float rcp) {
-    // (o - org) * rcp spans [0, 65000] over the bounds; 2^8 cells
+    // (o - org) * rcp spans [2, 65002] over the bounds; 2^8 cells
     const float q = (o - org) * rcp * (256.0f / 65000.0f);
     return q >= 255.0f ? 255u : (q > 0.0f ? (uint32_t)q : 0u);
 }

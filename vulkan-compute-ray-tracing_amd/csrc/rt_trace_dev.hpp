@@ -67,16 +67,21 @@ __device__ __forceinline__ bool slab(float lox, float loy, float loz, float hix,
     return slab_accept(nearv, farv);
 }
 
-// The same test with each axis' planes already in the order the ray meets
-// them: (nx, ny, nz) the planes it enters by, (fx, fy, fz) those it leaves
-// by.  For inv >= 0 that is the low plane first, for inv < 0 the high one;
-// rounding is monotonic, so these t are exactly slab()'s min and max and the
-// two tests decide alike, with four min/max fewer per box.
+// Slab test of a 16-bit box in the frame's units with each axis' planes in
+// the order the ray meets them: (nx, ny, nz) the planes it enters by,
+// (fx, fy, fz) those it leaves by (for inv >= 0 the low plane first, for
+// inv < 0 the high one), so no min/max pairs are needed.  A plane q is met at
+// t = q * inv + c with c = -bo * inv, one fused multiply-add.  Against the
+// exact (q - bo) * inv, the rounding of c moves a plane by |bo| * 2^-24
+// quanta and that of bo by |bo| * 2^-23; frame_setup keeps |bo| < 2^21, so
+// the planes move by under 0.4 quanta, inside the one-quantum padding every
+// quantized bound carries.  The remaining errors are relative (inv, the
+// final rounding) and slab_accept covers them.
 __device__ __forceinline__ bool slab_ordered(float nx, float ny, float nz, float fx, float fy, float fz,
-                                             const f3& o, const f3& inv, float tmin, float tmax, float& tnear) {
-    const float tnx = (nx - o.x) * inv.x, tfx = (fx - o.x) * inv.x;
-    const float tny = (ny - o.y) * inv.y, tfy = (fy - o.y) * inv.y;
-    const float tnz = (nz - o.z) * inv.z, tfz = (fz - o.z) * inv.z;
+                                             const f3& c, const f3& inv, float tmin, float tmax, float& tnear) {
+    const float tnx = __builtin_fmaf(nx, inv.x, c.x), tfx = __builtin_fmaf(fx, inv.x, c.x);
+    const float tny = __builtin_fmaf(ny, inv.y, c.y), tfy = __builtin_fmaf(fy, inv.y, c.y);
+    const float tnz = __builtin_fmaf(nz, inv.z, c.z), tfz = __builtin_fmaf(fz, inv.z, c.z);
     const float nearv = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
     const float farv = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
     tnear = nearv;
@@ -132,6 +137,41 @@ __device__ __forceinline__ float safe_inv(float d) {
     const float tiny = 0x1p-80f;
     float a = fabsf(d) < tiny ? copysignf(tiny, d) : d;
     return 1.0f / a;
+}
+
+// A ray origin at least this many quanta outside the 16-bit frame (32
+// frame extents) is moved along the ray first (frame_setup).
+constexpr float kFarOrigin = 2097152.0f;
+
+// Places a ray in a scene's 16-bit frame: a plane at q quanta is met at
+// t = t_shift + (q - bo) * inv.  Normally t_shift = 0 and bo is the origin's
+// position, (o - org) / scale in float.  The float rounding of bo moves every
+// plane by |bo| * 2^-23 quanta, which the one-quantum padding of quantized
+// bounds absorbs only while |bo| is small; so an origin at least kFarOrigin
+// quanta out is advanced, in double precision, to where the ray meets the
+// frame's box, and t_shift records how far.  Returns false when the ray
+// misses that box within [tmin, tmax]: it can hit no node box (a tree whose
+// root is a single triangle has no node box, and the frame bounds nothing).
+__device__ __forceinline__ bool frame_far(const RayIn& r, const QFrame& q, f3& bo, float& t_shift) {
+    const double o[3] = {((double)r.o.x - (double)q.ox) / (double)q.sx, ((double)r.o.y - (double)q.oy) / (double)q.sy,
+                         ((double)r.o.z - (double)q.oz) / (double)q.sz};
+    const double d[3] = {(double)r.d.x / (double)q.sx, (double)r.d.y / (double)q.sy, (double)r.d.z / (double)q.sz};
+    double t0 = (double)r.tmin, t1 = (double)r.tmax;
+    for (int k = 0; k < 3; ++k) {
+        // the box [-1, 65536] holds every quantized bound with room to spare
+        if (d[k] == 0.0) {
+            if (o[k] < -1.0 || o[k] > 65536.0) return false;
+            continue;
+        }
+        const double ta = (-1.0 - o[k]) / d[k], tb = (65536.0 - o[k]) / d[k];
+        t0 = fmax(t0, fmin(ta, tb));
+        t1 = fmin(t1, fmax(ta, tb));
+    }
+    if (!(t0 <= t1)) return false;
+    t_shift = (float)t0;
+    const double ts = (double)t_shift;
+    bo = f3{(float)(o[0] + ts * d[0]), (float)(o[1] + ts * d[1]), (float)(o[2] + ts * d[2])};
+    return true;
 }
 
 __device__ __forceinline__ uint32_t lane_id() {
@@ -228,6 +268,10 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     // 4-wide records: 16 where the ray runs toward -axis (its entry plane is
     // then the word's high half), so a rotate puts the entry plane low
     uint32_t rot_x = 0, rot_y = 0, rot_z = 0;
+    f3 cq{0.0f, 0.0f, 0.0f};        // -bo * inv: 4-wide box planes are q * inv + cq
+    // node tests run on t - t_shift (frame_setup); tmin_rel and best_rel are
+    // tmin and the closest hit's t on that scale (equal to them when t_shift = 0)
+    float t_shift = 0.0f, tmin_rel = 0.0f, best_rel = 0.0f;
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
     int32_t best = -1;
     uint32_t nv = 0, nt = 0;
@@ -301,18 +345,28 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     }
                     inv = f3{safe_inv(r.d.x), safe_inv(r.d.y), safe_inv(r.d.z)};
                     bo = r.o;
+                    t_shift = 0.0f;
+                    bool reach = true;
                     if (LAYOUT == kLayoutQ16 || LAYOUT == kLayoutW4) {
                         bo = f3{(r.o.x - q.ox) * q.rx, (r.o.y - q.oy) * q.ry, (r.o.z - q.oz) * q.rz};
                         inv = f3{inv.x * q.sx, inv.y * q.sy, inv.z * q.sz};
+                        if (!(fmaxf(fmaxf(fabsf(bo.x), fabsf(bo.y)), fabsf(bo.z)) < kFarOrigin))
+                            reach = frame_far(r, q, bo, t_shift);
                     }
                     rot_x = (__float_as_uint(inv.x) >> 27) & 16u;
                     rot_y = (__float_as_uint(inv.y) >> 27) & 16u;
                     rot_z = (__float_as_uint(inv.z) >> 27) & 16u;
+                    cq = f3{-(bo.x * inv.x), -(bo.y * inv.y), -(bo.z * inv.z)};
+                    tmin_rel = r.tmin - t_shift;
+                    best_rel = r.tmax - t_shift;
                     best_t = r.tmax; bu = 0.0f; bv = 0.0f; best = -1;
                     nv = 0; nt = 0; sp = 0;
                     cur = root;
                     busy = true;
-                    if (root == kEmptyRoot) cur = 0, sp = -1;   // finishes on the first step
+                    // an empty tree, or a ray that misses the frame's box
+                    // (which bounds every node box, so only a tree with an
+                    // internal root), finishes on the first step
+                    if (root == kEmptyRoot || (!reach && root >= 0)) cur = 0, sp = -1;
                 }
             }
         }
@@ -352,8 +406,8 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                         const uint32_t x = __builtin_amdgcn_alignbit(wx, wx, rot_x);
                         const uint32_t y = __builtin_amdgcn_alignbit(wy, wy, rot_y);
                         const uint32_t z = __builtin_amdgcn_alignbit(wz, wz, rot_z);
-                        if (slab_ordered(lo16(x), lo16(y), lo16(z), hi16(x), hi16(y), hi16(z), bo, inv, r.tmin,
-                                         best_t, tn)) {
+                        if (slab_ordered(lo16(x), lo16(y), lo16(z), hi16(x), hi16(y), hi16(z), cq, inv, tmin_rel,
+                                         best_rel, tn)) {
                             key = tn;
                             nh++;
                         }
@@ -389,6 +443,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     float t, u, v;
                     if (tri_test(tr, r.o, r.d, r.tmin, best_t, t, u, v)) {
                         best_t = t; bu = u; bv = v; best = slot;
+                        best_rel = t - t_shift;
                     }
                 }
             } else if (sp >= 0) {
@@ -400,17 +455,17 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                         const uint4* np = reinterpret_cast<const uint4*>(sc.qnodes + cur);
                         uint4 a = np[0], b = np[1];
                         h0 = slab(lo16(a.x), hi16(a.x), lo16(a.y), hi16(a.y), lo16(a.z), hi16(a.z), bo, inv,
-                                  r.tmin, best_t, tn0);
+                                  tmin_rel, best_rel, tn0);
                         h1 = slab(lo16(a.w), hi16(a.w), lo16(b.x), hi16(b.x), lo16(b.y), hi16(b.y), bo, inv,
-                                  r.tmin, best_t, tn1);
+                                  tmin_rel, best_rel, tn1);
                         c0 = (int32_t)b.z;
                         c1 = (int32_t)b.w;
                     } else {
                         const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
                         float4 a = np[0], b = np[1], c = np[2];
                         int4 dd = reinterpret_cast<const int4*>(np)[3];
-                        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, bo, inv, r.tmin, best_t, tn0);
-                        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, bo, inv, r.tmin, best_t, tn1);
+                        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, bo, inv, tmin_rel, best_rel, tn0);
+                        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, bo, inv, tmin_rel, best_rel, tn1);
                         c0 = dd.x;
                         c1 = dd.y;
                     }
@@ -432,6 +487,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     float t, u, v;
                     if (tri_test(tr, r.o, r.d, r.tmin, best_t, t, u, v)) {
                         best_t = t; bu = u; bv = v; best = slot;
+                        best_rel = t - t_shift;
                     }
                 }
             }
