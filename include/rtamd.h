@@ -207,8 +207,8 @@ typedef struct rt_tuning {
                                traversals), 48 below (short ones)                */
     uint32_t blocks_per_cu; /* persistent blocks per CU (0 = occupancy limit)     */
     uint32_t stack_lds;     /* 4-wide layout: 0 = whole traversal stack in LDS;
-                               8 or 16 = that many top entries in LDS, deeper ones
-                               spill to HBM (more waves fit per CU)               */
+                               8, 16 or 32 = that many top entries in LDS, deeper
+                               ones spill to HBM (more waves fit per CU)          */
     uint32_t tree;          /* RT_TREE_*                                          */
     uint32_t ray_order;     /* RT_ORDER_*: the order a query's rays are traced in */
     uint32_t reserved[2];
@@ -233,7 +233,7 @@ typedef struct rt_batch_item {
 
 typedef struct rt_batch_config {
     uint32_t block;          /* threads per work-group: 128 or 256 (0 = 256)       */
-    uint32_t stack_lds;      /* 0 = whole stack in LDS; 8 or 16 = top entries only  */
+    uint32_t stack_lds;      /* 0 = whole stack in LDS; 8, 16 or 32 = top entries   */
     uint32_t regroup;        /* 1 = bin each item's rays by octant and origin cell  */
     uint32_t refill;         /* idle lanes (1..64) before a wave fetches rays       */
     uint32_t blocks_per_cu;  /* 0 = occupancy limit                                 */

@@ -71,7 +71,7 @@ def test_batch_autotune_and_cache(rt, gpu_ctx, mixed, tmp_path):
     b = rt.Batch(gpu_ctx, items)
     cache = str(tmp_path / "tune.txt")
     best = b.autotune(trials=2, cache_path=cache)
-    assert best.block in (128, 256) and best.stack_lds in (0, 8, 16) and 1 <= best.refill <= 64
+    assert best.block in (128, 256) and best.stack_lds in (0, 8, 16, 32) and 1 <= best.refill <= 64
     assert os.path.getsize(cache) > 0
     b.destroy()
     b2 = rt.Batch(gpu_ctx, items)
@@ -90,7 +90,7 @@ def test_plain_ring_stack_matches_full_stack(rt, gpu_ctx):
     rays = rt.rays_to_device(rt.rays_box(50000, 3, 4.0))
     print("case: plain ring stacks", flush=True)
     ref = rt.hits_from_device(gpu_ctx.closest_hit(scene, rays))
-    for ring in (8, 16):
+    for ring in (8, 16, 32):
         gpu_ctx.set_tuning(stack_lds=ring)
         try:
             got = rt.hits_from_device(gpu_ctx.closest_hit(scene, rays))

@@ -124,7 +124,7 @@ rt_batch_config default_config() {
 }
 
 bool valid_config(const rt_batch_config& c) {
-    return (c.block == 128 || c.block == 256) && (c.stack_lds == 0 || c.stack_lds == 8 || c.stack_lds == 16) &&
+    return (c.block == 128 || c.block == 256) && (c.stack_lds == 0 || c.stack_lds == 8 || c.stack_lds == 16 || c.stack_lds == 32) &&
            c.regroup <= 1 && c.refill >= 1 && c.refill <= 64 && c.blocks_per_cu <= 16 && c.use_graph <= 1;
 }
 
@@ -195,6 +195,7 @@ template <int BLOCK, int STACK>
 int launch_batch_ring(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
     if (b->cfg.stack_lds == 8 && STACK > 8) return launch_batch_kernel<BLOCK, STACK, 8>(ctx, b, s);
     if (b->cfg.stack_lds == 16 && STACK > 16) return launch_batch_kernel<BLOCK, STACK, 16>(ctx, b, s);
+    if (b->cfg.stack_lds == 32 && STACK > 32) return launch_batch_kernel<BLOCK, STACK, 32>(ctx, b, s);
     return launch_batch_kernel<BLOCK, STACK, 0>(ctx, b, s);
 }
 
@@ -491,7 +492,7 @@ extern "C" int rt_batch_autotune(rt_ctx* ctx, rt_batch* b, uint32_t trials, cons
     rt_batch_config cur = b->cfg;
     float cur_ms = 0.0f;
     int rc = time_cfg(cur, cur_ms);
-    const std::vector<uint32_t> blocks = {256, 128}, rings = {0, 16, 8}, regroups = {0, 1},
+    const std::vector<uint32_t> blocks = {256, 128}, rings = {0, 32, 16, 8}, regroups = {0, 1},
                                 refills = {8, 16, 32, 48, 64}, bpcs = {0, 2, 3, 4};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
         for (int p = 0; p < 5 && !rc; ++p) {

@@ -232,6 +232,7 @@ template <int STACK>
 int launch_w4(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
     const uint32_t ring = ctx->tuning.stack_lds;
     if (ring == 8 && STACK > 8) return launch_packed_t<STACK, kLayoutW4, 8>(ctx, a, s, stats);
+    if (ring == 32 && STACK > 32) return launch_packed_t<STACK, kLayoutW4, 32>(ctx, a, s, stats);
     if (ring == 16 && STACK > 16) return launch_packed_t<STACK, kLayoutW4, 16>(ctx, a, s, stats);
     return launch_packed_t<STACK, kLayoutW4, 0>(ctx, a, s, stats);
 }
