@@ -169,7 +169,7 @@ typedef struct rt_query_params {
 } rt_query_params;
 
 /* hit_bvh (ray-trace-compute.comp:263-311) over n rays: rays_dev[n] ->
- * hits_dev[n].  Enqueued on `stream`; returns without waiting. */
+ * hits_dev[n], n < 2^31.  Enqueued on `stream`; returns without waiting. */
 int rt_closest_hit(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays_dev,
                    uint32_t n, rt_hit* hits_dev, const rt_query_params* params,
                    void* stream);
@@ -369,8 +369,8 @@ enum { RT_SERVE_BATCHED = 0, RT_SERVE_FIFO = 1 };
 typedef struct rt_server_config {
     uint32_t mode;            /* RT_SERVE_*                                     */
     uint32_t slots;           /* concurrent device launches (1..8, 0 = 4)       */
-    uint32_t max_batch_rays;  /* rays per batched launch (0 = 1,048,576)        */
-    uint32_t max_render_pixels; /* largest render request (0 = 1,048,576)       */
+    uint32_t max_batch_rays;  /* rays per batched launch (0 = 1,048,576; <= 2^30) */
+    uint32_t max_render_pixels; /* largest render request (0 = 1,048,576; <= 2^30) */
     uint64_t hbm_budget;      /* bytes for resident scenes (0 = no limit)       */
     uint32_t reserved[4];
 } rt_server_config;

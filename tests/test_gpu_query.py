@@ -281,6 +281,26 @@ def test_invalid_scene_rejected_by_upload(rt, gpu_ctx):
     assert e.value.status == rtamd.RT_ERR_INVALID and "child index" in str(e.value)
 
 
+def test_ray_count_limits(rt, gpu_ctx):
+    """Ray and slot counts at 2^31 and above are refused before any device
+    work (the kernels' 32-bit grid-stride indices need fewer).  The buffers
+    are null, which would be refused too, so nothing is ever launched."""
+    import ctypes
+    import rtamd
+    L = rtamd.lib()
+    hs = rt.scene_soup(16, 1)
+    scene = gpu_ctx.upload(hs)
+    print("case: ray count limits", flush=True)
+    for n in (1 << 31, 0xffffffff):
+        rc = L.rt_closest_hit(gpu_ctx.handle, scene.handle, None, n, None, None, None)
+        assert rc == rtamd.RT_ERR_INVALID
+        assert b"2^31" in L.rt_last_error(gpu_ctx.handle)
+    with pytest.raises(rtamd.RtError) as e:
+        rtamd.Server(gpu_ctx, max_batch_rays=(1 << 30) + 1)
+    assert e.value.status == rtamd.RT_ERR_INVALID and "2^30" in str(e.value)
+    scene.release()
+
+
 def test_trace_order_is_invisible(rt, gpu_ctx):
     """RT_ORDER_SORTED traces the rays in Morton order of their origins:
     hits and per-ray counters are byte-identical to index order."""

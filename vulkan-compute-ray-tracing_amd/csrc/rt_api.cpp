@@ -126,6 +126,8 @@ int query_common(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays, uint32_
     if (!ctx) return RT_ERR_INVALID;
     if (!scene_usable(ctx, scene)) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: scene not on this context's device");
     if (n == 0) return RT_OK;
+    // the kernels' 32-bit grid-stride indices stay below 2^32 for n < 2^31
+    if (n > 0x7fffffffu) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: more than 2^31 - 1 rays in one call");
     if (!rays || !hits) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: null ray or hit buffer");
     if (!aligned16(rays) || !aligned16(hits)) return set_error(ctx, RT_ERR_INVALID, "rt_closest_hit: buffers must be 16-byte aligned");
     rt_query_params p{};

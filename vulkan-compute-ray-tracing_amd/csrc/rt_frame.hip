@@ -70,7 +70,8 @@ extern "C" int rt_postprocess_gamma(rt_ctx* ctx, const uint8_t* src_dev, uint8_t
     if (!src_dev || !dst_dev) return set_error(ctx, RT_ERR_INVALID, "rt_postprocess_gamma: null image");
     if (((uintptr_t)src_dev & 3u) || ((uintptr_t)dst_dev & 3u))
         return set_error(ctx, RT_ERR_INVALID, "rt_postprocess_gamma: images must be 4-byte aligned");
-    if (count > 0xffffffffull) return set_error(ctx, RT_ERR_INVALID, "rt_postprocess_gamma: image too large");
+    // the pass's 32-bit grid-stride index stays below 2^32 for fewer than 2^31 pixels
+    if (count > 0x7fffffffull) return set_error(ctx, RT_ERR_INVALID, "rt_postprocess_gamma: image too large");
     static const GammaLut lut = make_gamma_lut();
     RT_HIP(ctx, hipSetDevice(ctx->device));
     const uint32_t grid = (uint32_t)std::min<uint64_t>((count + 255) / 256, (uint64_t)ctx->num_cus * 16u);

@@ -373,6 +373,9 @@ extern "C" int rt_server_create(rt_ctx* ctx, const rt_server_config* config, rt_
     if (c.mode == RT_SERVE_FIFO) c.slots = 1;
     if (c.max_batch_rays == 0) c.max_batch_rays = 1u << 20;
     if (c.max_render_pixels == 0) c.max_render_pixels = 1u << 20;
+    // a slot's batch is one launch: the kernels' 32-bit indices need < 2^31 rays
+    if (c.max_batch_rays > (1u << 30) || c.max_render_pixels > (1u << 30))
+        return set_error(ctx, RT_ERR_INVALID, "rt_server_create: max_batch_rays and max_render_pixels are at most 2^30");
     RT_HIP(ctx, hipSetDevice(ctx->device));
     rt_server* S = new rt_server();
     S->ctx = ctx;
