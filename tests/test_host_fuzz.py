@@ -1,0 +1,62 @@
+"""The library's host-side parsers and validators (OBJ reader, scene check,
+light rule, host BVH builder, image writer) under AddressSanitizer and
+UBSan, fed mutated and random inputs by tests/cpp/host_fuzz.cpp.  The host
+sources are rebuilt with g++ and the sanitizers; the HIP objects come
+unchanged from the library build and are never called.  CPU only."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "vulkan-compute-ray-tracing_amd")
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+HOST = ["rt_scene", "rt_obj", "rt_bvh", "rt_image", "rt_workloads"]
+DEVICE = ["rt_api", "rt_trace", "rt_render", "rt_batch", "rt_build", "rt_frame", "rt_sphere", "rt_server"]
+SAN = ["-fsanitize=address,undefined,float-cast-overflow", "-fno-omit-frame-pointer"]
+
+pytestmark = pytest.mark.slow
+
+
+@pytest.fixture(scope="module")
+def fuzzer(rt, tmp_path_factory):
+    rt.lib()
+    gxx = shutil.which("g++")
+    rocm_inc = "/opt/rocm/include"
+    if gxx is None or not os.path.isdir(rocm_inc):
+        pytest.skip("g++ or the ROCm headers are not present")
+    if not all(os.path.exists(os.path.join(BUILD, f + ".o")) for f in DEVICE):
+        pytest.skip("library objects not built")
+    out = tmp_path_factory.mktemp("fuzz")
+    flags = ["-std=c++17", "-O1", "-g", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I", rocm_inc,
+             "-I", os.path.join(ROOT, "include"), "-I", CSRC] + SAN
+    objs = []
+    jobs = []
+    for f in HOST:
+        o = str(out / (f + ".o"))
+        jobs.append(subprocess.Popen([gxx] + flags + ["-c", os.path.join(CSRC, f + ".cpp"), "-o", o],
+                                     stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+        objs.append(o)
+    main_o = str(out / "host_fuzz.o")
+    jobs.append(subprocess.Popen([gxx] + flags + ["-c", os.path.join(ROOT, "tests", "cpp", "host_fuzz.cpp"), "-o",
+                                                  main_o], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+    for j in jobs:
+        log, _ = j.communicate(timeout=600)
+        assert j.returncode == 0, log
+    exe = str(out / "host_fuzz")
+    cmd = [gxx] + SAN + ["-o", exe, main_o] + objs + [os.path.join(BUILD, f + ".o") for f in DEVICE] + \
+          ["-L/opt/rocm/lib", "-lamdhip64", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe, str(out)
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_host_inputs_under_sanitizers(fuzzer, seed):
+    exe, work = fuzzer
+    env = dict(os.environ, UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1", ASAN_OPTIONS="detect_leaks=1")
+    r = subprocess.run([exe, str(seed), "600", work], capture_output=True, text=True, timeout=600, env=env)
+    assert r.returncode == 0 and "ERROR" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
+    assert f"fuzz seed {seed}: 600 iterations" in r.stdout

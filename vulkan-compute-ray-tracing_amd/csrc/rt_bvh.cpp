@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <system_error>
 #include <thread>
 #include <vector>
 
