@@ -183,6 +183,7 @@ int launch_slot(rt_server* S, Slot& sl) {
             return RT_ERR_HIP;
     } else {
         uint32_t off = 0, need = 0;
+        uint64_t long_rays = 0;     // rays on scenes of 2^17 triangles or more
         const uint32_t ni = (uint32_t)sl.reqs.size();
         for (uint32_t i = 0; i < ni; ++i) {
             Request* r = sl.reqs[i];
@@ -212,6 +213,7 @@ int launch_slot(rt_server* S, Slot& sl) {
             sl.h_offsets[i] = off;
             off += r->n;
             need = std::max(need, 3u * sc->height4 + 1u);
+            if (sc->info.num_triangles >= (1u << 17)) long_rays += r->n;
         }
         sl.h_offsets[ni] = off;
         const uint32_t chunks = (off + 63) / 64;
@@ -222,7 +224,10 @@ int launch_slot(rt_server* S, Slot& sl) {
             hipMemcpyAsync(sl.d_chunk, sl.h_chunk, std::max(chunks, 1u) * 4, hipMemcpyHostToDevice, st) != hipSuccess)
             return RT_ERR_HIP;
         BatchIo bio{sl.d_table, sl.d_offsets, sl.d_chunk, nullptr};
-        int rc = launch_batch_dynamic(sl.ctx, bio, sl.d_cursor, off, need, 32u, st);
+        // refill threshold as for a plain query: early refills when most rays
+        // trace a large scene (long traversals), late ones otherwise
+        const uint32_t refill = 2ull * long_rays >= off ? 16u : 48u;
+        int rc = launch_batch_dynamic(sl.ctx, bio, sl.d_cursor, off, need, refill, st);
         if (rc) return rc;
         for (Request* r : sl.reqs) {
             rt_hit* dst = r->direct ? r->hits : sl.h_hits + r->offset;
