@@ -329,8 +329,10 @@ static int box_interval(v3 o, v3 invd, const o_node* nd, float tmin, float tmax,
     float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
     float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
     *tn = n;
-    /* a relative slack of 2^-16 keeps the test conservative under rounding */
-    return n <= f + (fabsf(f) + fabsf(n)) * 0x1p-16f;
+    /* near - |near| * 2^-15 <= far (one fused multiply-add): a relative
+       slack that keeps the test conservative under rounding, the same
+       accept test as the packed kernels' */
+    return fmaf(-fabsf(n), 0x1p-15f, n) <= f;
 }
 
 #define ORC_ORDERED_STACK 4096
