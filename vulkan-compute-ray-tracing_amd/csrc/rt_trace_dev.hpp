@@ -73,7 +73,7 @@ __device__ __forceinline__ bool slab(float lox, float loy, float loz, float hix,
 // inv < 0 the high one), so no min/max pairs are needed.  A plane q is met at
 // t = q * inv + c with c = -bo * inv, one fused multiply-add.  Against the
 // exact (q - bo) * inv, the rounding of c moves a plane by |bo| * 2^-24
-// quanta and that of bo by |bo| * 2^-23; frame_setup keeps |bo| < 2^21, so
+// quanta and that of bo by |bo| * 2^-23; frame_far keeps |bo| < 2^21, so
 // the planes move by under 0.4 quanta, inside the one-quantum padding every
 // quantized bound carries.  The remaining errors are relative (inv, the
 // final rounding) and slab_accept covers them.
@@ -140,7 +140,7 @@ __device__ __forceinline__ float safe_inv(float d) {
 }
 
 // A ray origin at least this many quanta outside the 16-bit frame (32
-// frame extents) is moved along the ray first (frame_setup).
+// frame extents) is moved along the ray first (frame_far).
 constexpr float kFarOrigin = 2097152.0f;
 
 // Places a ray in a scene's 16-bit frame: a plane at q quanta is met at
@@ -223,7 +223,8 @@ __device__ __forceinline__ float hi16(uint32_t w) { return (float)(w >> 16); }
 // triangle record per busy lane.  With 16-bit nodes the ray is moved once
 // into the nodes' fixed-point frame (origin (o - org) / scale, reciprocal
 // direction times scale): slab distances t are the same numbers, so tmin,
-// tmax and the closest t apply unchanged.
+// tmax and the closest t apply unchanged -- except for a ray from far
+// outside the frame, whose node tests run on t - t_shift (frame_far).
 //
 // BATCHED: rays come from many items (BatchIo); each lane carries its own
 // scene pointers.  RING > 0: only the top RING stack entries of a lane live
@@ -247,7 +248,8 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     // contiguous pieces, each with its own cursor (io.cursor[k * kCursorStride]).
     // Blocks are dispatched to the 8 XCDs round robin, so each XCD's blocks
     // start on one run of adjacent segments: each XCD's L2 then serves one
-    // region of the (sorted) rays, and the cursors' atomics are spread out.  A wave whose segment is used up moves to the next one, and it is
+    // region of the (sorted) rays, and the cursors' atomics are spread out.
+    // A wave whose segment is used up moves to the next one, and it is
     // drained once it has found all of them used up.  Correctness does not
     // depend on the dispatch order, only locality does.
     // block b runs on XCD b % kXcds; with kSegments segments each XCD starts
@@ -269,7 +271,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     // then the word's high half), so a rotate puts the entry plane low
     uint32_t rot_x = 0, rot_y = 0, rot_z = 0;
     f3 cq{0.0f, 0.0f, 0.0f};        // -bo * inv: 4-wide box planes are q * inv + cq
-    // node tests run on t - t_shift (frame_setup); tmin_rel and best_rel are
+    // node tests run on t - t_shift (frame_far); tmin_rel and best_rel are
     // tmin and the closest hit's t on that scale (equal to them when t_shift = 0)
     float t_shift = 0.0f, tmin_rel = 0.0f, best_rel = 0.0f;
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
