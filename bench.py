@@ -357,11 +357,13 @@ def bench_small_and_medium(ctx, threads: int):
 DOGE_TRIANGLES = os.path.join(ROOT, "tests", "golden", "doge_triangles.npy")
 
 
-def bench_doge_frame_loop(ctx, frames: int = 60):
+def bench_doge_frame_loop(ctx, frames: int = 60, device_tree: bool = True):
     """The reference application's own workload: its doge scene (1,422
     triangles, RtScene.h:44-101) in an 800x600 window with the shader's vfov 30
     and two bounces, camera of main.cpp:37, one compute pass plus the display
-    pass per frame -- the frame whose time main.cpp:406-412 prints."""
+    pass per frame -- the frame whose time main.cpp:406-412 prints.  With
+    device_tree, the same frame on the device-built tree (RT_TREE_DEVICE) is
+    timed as well."""
     import numpy as np
     import torch
     import rtamd
@@ -401,6 +403,13 @@ def bench_doge_frame_loop(ctx, frames: int = 60):
            "frames_per_s": round(1e3 / wall_ms, 1), "rays_per_frame": prim + sec,
            "mrays_per_s": round((prim + sec) / wall_ms / 1e3, 1)}
     log(f"doge frame loop: {wall_ms:.3f} ms/frame ({out['frames_per_s']} frames/s)")
+    if device_tree:
+        t = ctx.get_tuning()
+        ctx.set_tuning(t.node_layout, t.refill, t.blocks_per_cu, t.stack_lds, rtamd.TREE_DEVICE, t.ray_order)
+        try:
+            out["ms_per_frame_device_tree"] = bench_doge_frame_loop(ctx, frames, device_tree=False)["ms_per_frame"]
+        finally:
+            ctx.set_tuning(t.node_layout, t.refill, t.blocks_per_cu, t.stack_lds, t.tree, t.ray_order)
     return out
 
 
