@@ -555,6 +555,31 @@ def main():
                          "primary_trace_mrays_per_s": round(prim / st.trace_primary_ms / 1e3, 1),
                          "secondary_trace_mrays_per_s": round(sec / st.trace_secondary_ms / 1e3, 1)}
         log(f"render: {rms:.1f} ms, secondary traversal {out['render']['secondary_trace_mrays_per_s']} Mrays/s")
+        if not args.plain_only:
+            # the same render on the device-built tree (RT_TREE_DEVICE): its
+            # time, and how far its image is from the given tree's (only the
+            # choice between exactly equal hits can differ)
+            given = tgt.cpu().numpy().astype("float64")
+            t = ctx.get_tuning()
+            ctx.set_tuning(t.node_layout, t.refill, t.blocks_per_cu, t.stack_lds, rtamd.TREE_DEVICE, t.ray_order)
+            try:
+                dscene = ctx.upload(hs)
+            finally:
+                ctx.set_tuning(t.node_layout, t.refill, t.blocks_per_cu, t.stack_lds, t.tree, t.ray_order)
+            tgt.zero_()
+            acc.zero_()
+            ctx.render_frames(dscene, [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8, target=tgt, accum=acc)
+            torch.cuda.synchronize()
+            e0.record()
+            ctx.render_frames(dscene, [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8, target=tgt, accum=acc)
+            e1.record()
+            torch.cuda.synchronize()
+            d = (tgt.cpu().numpy().astype("float64") - given) / 255.0
+            out["render"]["device_tree_ms"] = round(e0.elapsed_time(e1), 3)
+            out["render"]["device_tree_rmse"] = float("%.3g" % float((d * d).mean(axis=(0, 1)).max() ** 0.5))
+            log(f"render on the device-built tree: {out['render']['device_tree_ms']} ms, "
+                f"rmse {out['render']['device_tree_rmse']} against the given tree's image")
+            dscene.release()
 
     if not args.plain_only:
         out["device_build"] = bench_device_build(ctx, hs, rays, threads, layout, node_bytes)
