@@ -74,7 +74,9 @@ constexpr int32_t kNoChild = INT32_MIN + 1;
 // Fixed-point frame of the quantized nodes, with 1/scale for the rays.
 struct QFrame { float ox, oy, oz, sx, sy, sz, rx, ry, rz; };
 
-enum { kLayoutF32 = 1, kLayoutQ16 = 2, kLayoutW4 = 3 };
+// kLayoutW4Joined is kLayoutW4 with its records and packed triangles in one
+// allocation (rt_scene::tri_off), addressed by 32-bit offsets.
+enum { kLayoutF32 = 1, kLayoutQ16 = 2, kLayoutW4 = 3, kLayoutW4Joined = 4 };
 
 // One item of a batched launch: a resident scene's 4-wide tree and a ray
 // batch with its hit buffer.  Global ray g of the launch belongs to the item
@@ -100,6 +102,7 @@ struct BatchIo {
 
 
 constexpr int32_t kEmptyRoot = INT32_MIN;  // tree with no reachable triangle
+constexpr uint32_t kSeparateArrays = 0xffffffffu;   // rt_scene::tri_off: not one allocation
 
 // Bytes each traversal step reads, used for the roofline accounting.
 constexpr uint32_t kNodeBytes = 64;
@@ -166,6 +169,10 @@ struct rt_scene {
     int32_t root4 = rtamd::kEmptyRoot;
     uint32_t height4 = 0;
     rtamd::PTri* d_ptris = nullptr;
+    // d_w4nodes and d_ptris share one allocation when both fit in 4 GiB:
+    // d_ptris = d_w4nodes + tri_off bytes, so a kernel addresses either
+    // record by a 32-bit offset from one base.  kSeparateArrays otherwise.
+    uint32_t tri_off = rtamd::kSeparateArrays;
     int32_t* d_ptri_id = nullptr;
     int32_t root_ref = rtamd::kEmptyRoot;
     uint32_t packed_height = 0;

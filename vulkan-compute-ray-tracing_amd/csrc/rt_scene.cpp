@@ -323,6 +323,30 @@ int upload_array(rt_ctx* ctx, T** dst, const T* src, uint32_t n, uint64_t& bytes
     return RT_OK;
 }
 
+// The 4-wide node records and the packed triangles, in one allocation when
+// they fit in 4 GiB (nodes first, triangles from the next 256-byte boundary;
+// the traversal then addresses both by 32-bit offsets from one base), else
+// in two.
+int upload_records(rt_ctx* ctx, rt_scene* s, const PackedTree& packed, uint64_t& bytes) {
+    const size_t nb = (sizeof(PNode4) * packed.w4nodes.size() + 255) / 256 * 256;
+    const size_t tb = sizeof(PTri) * packed.tris.size();
+    if (nb + tb >= (1ull << 32) || !nb || !tb) {
+        int rc = upload_array(ctx, &s->d_w4nodes, packed.w4nodes.data(), (uint32_t)packed.w4nodes.size(), bytes);
+        if (!rc) rc = upload_array(ctx, &s->d_ptris, packed.tris.data(), (uint32_t)packed.tris.size(), bytes);
+        s->tri_off = kSeparateArrays;
+        return rc;
+    }
+    char* base = nullptr;
+    RT_HIP(ctx, hipMalloc((void**)&base, nb + tb));
+    s->d_w4nodes = reinterpret_cast<PNode4*>(base);          // owns the allocation
+    s->d_ptris = reinterpret_cast<PTri*>(base + nb);
+    s->tri_off = (uint32_t)nb;
+    RT_HIP(ctx, hipMemcpy(base, packed.w4nodes.data(), sizeof(PNode4) * packed.w4nodes.size(), hipMemcpyHostToDevice));
+    RT_HIP(ctx, hipMemcpy(base + nb, packed.tris.data(), tb, hipMemcpyHostToDevice));
+    bytes += nb + tb;
+    return RT_OK;
+}
+
 // Sphere tree: median split on the longest axis of the centres' bounds
 // (ties by sphere index), one sphere per leaf, boxes = the spheres' bounds
 // padded by 2^-10 (|centre| + radius) + 1e-6 so rounding in the sphere
@@ -390,9 +414,11 @@ bool build_sphere_tree(const rt_sphere* sp, uint32_t n, std::vector<SphereNode>&
 
 void free_scene(rt_scene* s) {
     if (!s) return;
+    // d_ptris lies inside d_w4nodes' allocation unless the two are separate
+    void* ptris = s->tri_off == kSeparateArrays ? (void*)s->d_ptris : nullptr;
     for (void* p : {(void*)s->d_tris, (void*)s->d_mats, (void*)s->d_nodes, (void*)s->d_lights,
                     (void*)s->d_spheres, (void*)s->d_pnodes, (void*)s->d_qnodes, (void*)s->d_w4nodes,
-                    (void*)s->d_ptris, (void*)s->d_snodes,
+                    ptris, (void*)s->d_snodes,
                     (void*)s->d_ptri_id})
         if (p) (void)hipFree(p);
     delete s;
@@ -492,8 +518,7 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene*
         if (fast) {
             if ((rc = upload_array(ctx, &s->d_pnodes, packed.nodes.data(), (uint32_t)packed.nodes.size(), bytes))) break;
             if ((rc = upload_array(ctx, &s->d_qnodes, packed.qnodes.data(), (uint32_t)packed.qnodes.size(), bytes))) break;
-            if ((rc = upload_array(ctx, &s->d_w4nodes, packed.w4nodes.data(), (uint32_t)packed.w4nodes.size(), bytes))) break;
-            if ((rc = upload_array(ctx, &s->d_ptris, packed.tris.data(), (uint32_t)packed.tris.size(), bytes))) break;
+            if ((rc = upload_records(ctx, s, packed, bytes))) break;
             if ((rc = upload_array(ctx, &s->d_ptri_id, packed.tri_id.data(), (uint32_t)packed.tri_id.size(), bytes))) break;
         }
     } while (0);

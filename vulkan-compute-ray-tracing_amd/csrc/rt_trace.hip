@@ -215,7 +215,7 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
                                          : (sc->info.num_triangles >= (1u << 17) ? 16u : 48u);
     if (refill > 64u) refill = 64u;
     TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->frame,
-                  LAYOUT == kLayoutW4 ? sc->root4 : sc->root_ref};
+                  LAYOUT == kLayoutW4 || LAYOUT == kLayoutW4Joined ? sc->root4 : sc->root_ref, sc->tri_off};
     TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill, a.order, segs};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, ts, io, spill);
     RT_HIP(ctx, hipGetLastError());
@@ -234,6 +234,7 @@ int launch_w4(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
     if (ring == 8 && STACK > 8) return launch_packed_t<STACK, kLayoutW4, 8>(ctx, a, s, stats);
     if (ring == 32 && STACK > 32) return launch_packed_t<STACK, kLayoutW4, 32>(ctx, a, s, stats);
     if (ring == 16 && STACK > 16) return launch_packed_t<STACK, kLayoutW4, 16>(ctx, a, s, stats);
+    if (a.scene->tri_off != kSeparateArrays) return launch_packed_t<STACK, kLayoutW4Joined, 0>(ctx, a, s, stats);
     return launch_packed_t<STACK, kLayoutW4, 0>(ctx, a, s, stats);
 }
 

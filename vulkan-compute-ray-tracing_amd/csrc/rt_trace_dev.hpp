@@ -187,6 +187,7 @@ struct TraceScene {
     const int32_t* tri_id;
     QFrame frame;
     int32_t root;
+    uint32_t tri_off;         // kLayoutW4Joined: tris = w4nodes + tri_off bytes
 };
 struct TraceIo {
     const rt_ray* rays;
@@ -234,6 +235,7 @@ template <int BLOCK, int STACK, int LAYOUT, bool STATS, bool BATCHED, int RING>
 __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const TraceIo& io, const BatchIo& bio,
                                                  int32_t* __restrict__ stk, int32_t* __restrict__ spill) {
     static_assert(!BATCHED || LAYOUT == kLayoutW4, "batched launches use the 4-wide layout");
+    constexpr bool kW4 = LAYOUT == kLayoutW4 || LAYOUT == kLayoutW4Joined;
     static_assert(RING == 0 || (RING & (RING - 1)) == 0, "ring size is a power of two");
     const uint32_t n = io.n_dev ? *io.n_dev : io.n_host;
     const uint32_t tid = threadIdx.x;
@@ -349,7 +351,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     bo = r.o;
                     t_shift = 0.0f;
                     bool reach = true;
-                    if (LAYOUT == kLayoutQ16 || LAYOUT == kLayoutW4) {
+                    if (LAYOUT == kLayoutQ16 || kW4) {
                         bo = f3{(r.o.x - q.ox) * q.rx, (r.o.y - q.oy) * q.ry, (r.o.z - q.oz) * q.rz};
                         inv = f3{inv.x * q.sx, inv.y * q.sy, inv.z * q.sz};
                         if (!(fmaxf(fmaxf(fabsf(bo.x), fabsf(bo.y)), fabsf(bo.z)) < kFarOrigin))
@@ -379,17 +381,36 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
         // ---- one traversal step per busy lane
         if (busy) {
             bool next = false;          // cur holds the next record to read
-            if (sp >= 0 && LAYOUT == kLayoutW4) {
+            if (sp >= 0 && kW4) {
                 // One memory round trip per step whatever the lanes' mix of
                 // nodes and triangles: every lane issues its three 16-byte
                 // loads (node words 0-2 or the triangle record) and a fourth
                 // (node refs, or the triangle's first word again, so it never
                 // reads past the array) before any lane branches.
                 const bool node = cur >= 0;
-                const uint4* np = node ? reinterpret_cast<const uint4*>(l_w4 + cur)
-                                       : reinterpret_cast<const uint4*>(l_tris + ~cur);
-                const uint4 a = np[0], b = np[1], c = np[2];
-                uint4 dw = np[node ? 3 : 0];
+                uint4 a, b, c, dw;
+                if constexpr (LAYOUT == kLayoutW4Joined) {
+                    // one allocation: 32-bit byte offsets from its base, so
+                    // the loads take the base from scalar registers and no
+                    // lane computes a 64-bit address
+                    const char* base = reinterpret_cast<const char*>(sc.w4nodes);
+                    const uint32_t slot = (uint32_t)(~cur);
+                    const uint32_t off_node = (uint32_t)cur << 6;
+                    const uint32_t off_tri = sc.tri_off + (slot << 5) + (slot << 4);
+                    const uint32_t off = node ? off_node : off_tri;
+                    const uint32_t off_d = node ? off_node + 48u : off_tri;
+                    a = *reinterpret_cast<const uint4*>(base + off);
+                    b = *reinterpret_cast<const uint4*>(base + off + 16u);
+                    c = *reinterpret_cast<const uint4*>(base + off + 32u);
+                    dw = *reinterpret_cast<const uint4*>(base + off_d);
+                } else {
+                    const uint4* np = node ? reinterpret_cast<const uint4*>(l_w4 + cur)
+                                           : reinterpret_cast<const uint4*>(l_tris + ~cur);
+                    a = np[0];
+                    b = np[1];
+                    c = np[2];
+                    dw = np[node ? 3 : 0];
+                }
                 // Only the node branch reads dw, so the compiler would sink its
                 // load into that branch, after the triangle branch has already
                 // waited for the other three: two round trips for a wave that
