@@ -150,3 +150,24 @@ def test_upload_without_nodes(rt, gpu_ctx):
     got = rt.hits_from_device(gpu_ctx.closest_hit(sc, rt.rays_to_device(rays), traversal=rt.TRAVERSE_REFERENCE))
     assert np.array_equal(got["id"], want["id"])
     assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))
+
+
+def test_render_on_device_tree_matches_oracle(rt, gpu_ctx):
+    """The render on a device-built tree (RT_TREE_DEVICE): the shader's
+    paths on another tree, so the image equals the oracle's render on the
+    reference builder's tree up to exact-tie choices.  Bar: RMSE <= 1e-3 per
+    channel over two accumulated frames."""
+    hs = rt.scene_soup(1024, 1)
+    print("case: render 256x256 on the device tree", flush=True)
+    gpu_ctx.set_tuning(tree=rt.TREE_DEVICE)
+    try:
+        sc = gpu_ctx.upload(hs)
+    finally:
+        gpu_ctx.set_tuning()
+    tgt, _ = gpu_ctx.render_frames(sc, [0.0, 3.0, 0.0], 256, 256, 2, vfov=60.0, bounces=4)
+    torch.cuda.synchronize()
+    want, _ = O.render_frames(O.Scene(hs.triangles, hs.materials, hs.nodes, hs.lights, hs.spheres), [0.0, 3.0, 0.0],
+                              256, 256, 2, vfov=60.0, bounces=4, threads=8)
+    d = (tgt.cpu().numpy().astype(np.float64) - want.astype(np.float64)) / 255.0
+    assert float(np.sqrt((d * d).mean(axis=(0, 1))).max()) <= 1e-3
+    sc.release()
