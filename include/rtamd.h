@@ -211,8 +211,14 @@ typedef struct rt_tuning {
                                ones spill to HBM (more waves fit per CU)          */
     uint32_t tree;          /* RT_TREE_*                                          */
     uint32_t ray_order;     /* RT_ORDER_*: the order a query's rays are traced in */
-    uint32_t reserved[2];
+    uint32_t steal;         /* 4-wide layout, whole stack in LDS: once a wave's
+                               rays are used up, this many idle lanes (1..64)
+                               make it hand subtrees from its busy lanes' stacks
+                               to the idle ones (hits unchanged); 0 = 32,
+                               RT_STEAL_NEVER = never                            */
+    uint32_t reserved;
 } rt_tuning;
+enum { RT_STEAL_NEVER = 65 };
 int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t);
 int rt_tuning_get(const rt_ctx* ctx, rt_tuning* t);
 

@@ -54,7 +54,7 @@ def test_parameter_structs_match_the_header(rt):
              "BatchConfig": 32, "ServerConfig": 40, "SceneInfo": 48, "QueryStats": 24}
     for name, size in sizes.items():
         assert ctypes.sizeof(getattr(rt, name)) == size, name
-    assert rt.Tuning.tree.offset == 16 and rt.RenderStats.total_ms.offset == 24
+    assert rt.Tuning.tree.offset == 16 and rt.Tuning.steal.offset == 24 and rt.RenderStats.total_ms.offset == 24
 
 
 def test_oracle_layouts_agree(rt, orc):

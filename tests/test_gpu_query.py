@@ -370,6 +370,92 @@ def test_every_ray_traced_once_at_odd_sizes(rt, gpu_ctx, n):
         assert_equal_up_to_ties(oscene(hs), rays_np, got, ref)
 
 
+def tie_grid(rt, n=64, layers=3, seed=30):
+    """Unit squares on the planes z = 0, -1, ..., each cut into two right
+    triangles along alternating diagonals, in shuffled order.  Every normal
+    is (0, 0, +-1), so with dyadic rays the triangle test is exact: all
+    triangles of one plane give a ray the same t, and a ray through an edge
+    or a vertex has two or more hits at exactly equal t."""
+    i, j = np.meshgrid(np.arange(n, dtype=np.float32), np.arange(n, dtype=np.float32), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    flip = ((i + j) % 2 == 1)[:, None, None]
+    parts = []
+    for layer in range(layers):
+        z = np.full_like(i, -float(layer))
+        a, b = np.stack([i, j, z], 1), np.stack([i + 1, j, z], 1)
+        c, d = np.stack([i + 1, j + 1, z], 1), np.stack([i, j + 1, z], 1)
+        parts.append(np.where(flip, np.stack([a, b, c], 1), np.stack([a, b, d], 1)))
+        parts.append(np.where(flip, np.stack([a, c, d], 1), np.stack([b, c, d], 1)))
+    v = np.concatenate(parts)[np.random.default_rng(seed).permutation(2 * n * n * layers)]
+    t = np.zeros(len(v), rt.TRI_DT)
+    t["v0"], t["v1"], t["v2"] = v[:, 0], v[:, 1], v[:, 2]
+    return t
+
+
+def tie_rays(rt, m, n=64, seed=31):
+    """Dyadic rays from z = 4 to points on the grid's edges and vertices."""
+    rng = np.random.default_rng(seed)
+    r = np.zeros(m, rt.RAY_DT)
+    o = np.stack([rng.integers(0, 8 * n, m) / 8.0, rng.integers(0, 8 * n, m) / 8.0, np.full(m, 4.0)], 1)
+    kind = rng.integers(0, 3, m)
+    tx = rng.integers(1, n - 1, m).astype(np.float64)
+    ty = rng.integers(1, n - 1, m).astype(np.float64)
+    frac = rng.integers(0, 4, m) / 4.0
+    tx = np.where(kind == 0, tx + frac, tx)         # on a line y = integer
+    ty = np.where(kind == 1, ty + frac, ty)         # on a line x = integer; kind 2: a vertex
+    r["o"] = o
+    r["d"] = np.stack([tx - o[:, 0], ty - o[:, 1], np.full(m, -4.0)], 1)
+    r["tmin"] = 0.001
+    r["tmax"] = 10000.0
+    return r
+
+
+@pytest.mark.parametrize("case", ["tie_grid", "clusters"])
+def test_drain_work_sharing_is_invisible(rt, gpu_ctx, case):
+    """When a wave's rays run out, idle lanes traverse subtrees taken from
+    busy lanes' stacks and hand their hits back.  The hits must be the ones
+    of the whole traversal in one lane -- the counting variant, which never
+    shares -- byte for byte, exact ties included: on the tie grid most rays
+    end on an edge or a vertex shared by two or more triangles at equal t."""
+    if case == "tie_grid":
+        tris = tie_grid(rt)
+        mats = np.zeros(1, rt.MAT_DT)
+        hs = rt.HostScene(tris, mats, rt.bvh_build(tris, seed=1))
+        rays_np = tie_rays(rt, 200000)
+    else:
+        hs = rt.scene_clusters(1 << 18, 21)
+        rays_np = rt.rays_box(300000, 22, 4.0)
+    scene = gpu_ctx.upload(hs)
+    assert scene.info.fast_path == 1
+    rays = rt.rays_to_device(rays_np)
+    with pytest.raises(rt.RtError):
+        gpu_ctx.set_tuning(steal=rt.STEAL_NEVER + 1)
+    alone, _, _ = gpu_ctx.closest_hit_stats(scene, rays)
+    torch.cuda.synchronize()
+    alone = rt.hits_from_device(alone)
+    # steal 1: every idle lane takes work at once; 0: the default threshold
+    for steal in (0, 1, rt.STEAL_NEVER):
+        for mode in (rt.ORDER_GIVEN, rt.ORDER_SORTED):
+            print(f"case: drain work sharing, {case}, {len(rays_np)} rays, steal {steal}, order {mode}", flush=True)
+            gpu_ctx.set_tuning(ray_order=mode, steal=steal)
+            try:
+                shared = rt.hits_from_device(gpu_ctx.closest_hit(scene, rays))
+            finally:
+                gpu_ctx.set_tuning()
+            assert np.array_equal(shared.view(np.uint8), alone.view(np.uint8)), \
+                np.nonzero(shared["id"] != alone["id"])[0][:10]
+    if case == "tie_grid":
+        # every ray hits the top plane on an edge or a vertex; against the
+        # oracle's ordered traversal (2-wide visiting order) the ids may
+        # differ only by such a tie
+        assert (shared["id"] >= 0).all()
+        assert (shared["t"] == 1.0).all()
+        sample = rays_np[::50]
+        want = O.closest_hit(oscene(hs), sample, O.TRAV_ORDERED, threads=8)
+        ties = assert_equal_up_to_ties(oscene(hs), sample, shared[::50], want)
+        print(f"tie grid: {ties} of {len(sample)} sampled hits differ from the binary order by a tie", flush=True)
+
+
 def test_bench_configuration_against_the_oracle(rt, gpu_ctx):
     """The exact bench workload -- 4,194,304-triangle clustered soup (seed 3),
     4,194,304 incoherent rays (seed 4) -- through the default path (sorted

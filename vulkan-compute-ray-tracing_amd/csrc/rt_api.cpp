@@ -100,6 +100,8 @@ extern "C" int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t) {
     if (t->refill > 64) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: refill must be 0..64");
     if (t->tree > RT_TREE_DEVICE) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: unknown tree");
     if (t->ray_order > RT_ORDER_SORTED) return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: unknown ray order");
+    if (t->steal > RT_STEAL_NEVER)
+        return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: steal must be 0..64 or RT_STEAL_NEVER");
     if (t->stack_lds != 0 && t->stack_lds != 8 && t->stack_lds != 16 && t->stack_lds != 32)
         return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: stack_lds must be 0, 8, 16 or 32");
     ctx->tuning = *t;

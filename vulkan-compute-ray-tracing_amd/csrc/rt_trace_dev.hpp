@@ -200,6 +200,7 @@ struct TraceIo {
     uint32_t refill;
     const uint32_t* order;    // trace order (ray indices), or null: index order
     uint32_t segs;            // segments of the ray range (1 or kSegments), see below
+    uint32_t steal;           // idle lanes that make a drained wave share work (65: never)
 };
 
 __device__ __forceinline__ float lo16(uint32_t w) { return (float)(w & 0xffffu); }
@@ -216,6 +217,28 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& va, float& kb, int32_t
 }
 __device__ __forceinline__ float hi16(uint32_t w) { return (float)(w >> 16); }
 
+// Position of the n-th set bit (from 0) of m; n < popcount(m).  A binary
+// search on popcounts of the low halves.
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t n) {
+    const uint32_t lo = (uint32_t)m;
+    const uint32_t c = (uint32_t)__popc(lo);
+    uint32_t w = n < c ? lo : (uint32_t)(m >> 32);
+    uint32_t pos = n < c ? 0u : 32u;
+    n = n < c ? n : n - c;
+    for (uint32_t s = 16; s != 0; s >>= 1) {
+        const uint32_t cs = (uint32_t)__popc(w & ((1u << s) - 1u));
+        const bool up = n >= cs;
+        n = up ? n - cs : n;
+        w = up ? w >> s : w;
+        pos = up ? pos + s : pos;
+    }
+    return pos;
+}
+
+__device__ __forceinline__ float shfl_f(float x, uint32_t src) {
+    return __int_as_float(__shfl(__float_as_int(x), (int)src));
+}
+
 // Persistent traversal with per-lane ray fetch.  Each lane owns one ray at a
 // time; when at least `refill` lanes of the wave are idle, the wave takes
 // that many ray indices from the global cursor with one atomic and hands
@@ -231,11 +254,27 @@ __device__ __forceinline__ float hi16(uint32_t w) { return (float)(w >> 16); }
 // scene pointers.  RING > 0: only the top RING stack entries of a lane live
 // in LDS (a ring); deeper entries move to `spill` in HBM, one word per
 // push/pop beyond RING.  RING = 0 keeps the whole stack (STACK entries) in LDS.
+//
+// Work sharing in the drain (4-wide layouts, whole stack in LDS, no
+// counters): once a wave's rays are used up its idle lanes would wait for the
+// longest rays of the wave.  Instead each idle lane takes the bottom entry of
+// a busy lane's stack -- the subtree that lane would visit last -- and
+// traverses it for the ray's owner (a helper).  The ordered traversal's
+// visiting sequence is cut into pieces: a lane holds the piece
+// [plo, plo + 2^pk), and a steal halves the donor's piece, the donor keeping
+// the first half (its own remaining work) and the helper the second (the
+// stolen subtree, which comes after it).  Each piece accepts hits as the
+// whole traversal would (strictly closer than its best so far); a helper
+// starts from the donor's closest t, and at its end hands (t, piece, hit) to
+// the owner, who keeps the smaller t and on equal t the earlier piece.  The
+// result is the whole traversal's: the closest t, first in visiting order.
+// The owner writes it once its own piece and every helper have finished.
 template <int BLOCK, int STACK, int LAYOUT, bool STATS, bool BATCHED, int RING>
 __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const TraceIo& io, const BatchIo& bio,
                                                  int32_t* __restrict__ stk, int32_t* __restrict__ spill) {
     static_assert(!BATCHED || LAYOUT == kLayoutW4, "batched launches use the 4-wide layout");
     constexpr bool kW4 = LAYOUT == kLayoutW4 || LAYOUT == kLayoutW4Joined;
+    constexpr bool STEAL = kW4 && RING == 0 && !STATS && !BATCHED;
     static_assert(RING == 0 || (RING & (RING - 1)) == 0, "ring size is a power of two");
     const uint32_t n = io.n_dev ? *io.n_dev : io.n_host;
     const uint32_t tid = threadIdx.x;
@@ -281,6 +320,17 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     uint32_t nv = 0, nt = 0;
     int sp = 0;
     int32_t cur = 0;
+    // work sharing (STEAL): the stack holds entries [bot, sp); the lane's
+    // piece of its ray's visiting sequence is [plo, plo + 2^pk)
+    int bot = 0;
+    uint32_t plo = 0, pk = 31;
+    int32_t owner = -1;             // a helper's owner lane; -1 for the owner
+    uint32_t pending = 0;           // owner: helpers still running
+    uint32_t best_lo = 0;           // the piece that found `best`
+    // owner whose own piece is done while helpers still run: not busy (it
+    // takes no steps), not idle (it keeps its ray); only in a drained wave
+    bool waiting = false;
+    bool handoff = false;           // helper: done, result not yet handed over
     // the lane's scene (uniform unless BATCHED)
     const PNode4* l_w4 = sc.w4nodes;
     const PTri* l_tris = sc.tris;
@@ -304,6 +354,172 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
         int32_t v = stk[slot * BLOCK + tid];
         if (sp >= RING) stk[slot * BLOCK + tid] = spill[(size_t)(sp - RING) * gstride + gtid];
         return v;
+    };
+
+    // ---- one traversal step per busy lane; SH: with the drain's work sharing
+    auto step = [&](auto sh) {
+        constexpr bool SH = decltype(sh)::value;
+        if (!busy) return;
+        bool next = false;          // cur holds the next record to read
+        if (sp >= 0 && kW4) {
+            // One memory round trip per step whatever the lanes' mix of
+            // nodes and triangles: every lane issues its three 16-byte
+            // loads (node words 0-2 or the triangle record) and a fourth
+            // (node refs, or the triangle's first word again, so it never
+            // reads past the array) before any lane branches.
+            const bool node = cur >= 0;
+            uint4 a, b, c, dw;
+            if constexpr (LAYOUT == kLayoutW4Joined) {
+                // one allocation: 32-bit byte offsets from its base, so
+                // the loads take the base from scalar registers and no
+                // lane computes a 64-bit address
+                const char* base = reinterpret_cast<const char*>(sc.w4nodes);
+                const uint32_t slot = (uint32_t)(~cur);
+                const uint32_t off_node = (uint32_t)cur << 6;
+                const uint32_t off_tri = sc.tri_off + (slot << 5) + (slot << 4);
+                const uint32_t off = node ? off_node : off_tri;
+                const uint32_t off_d = node ? off_node + 48u : off_tri;
+                a = *reinterpret_cast<const uint4*>(base + off);
+                b = *reinterpret_cast<const uint4*>(base + off + 16u);
+                c = *reinterpret_cast<const uint4*>(base + off + 32u);
+                dw = *reinterpret_cast<const uint4*>(base + off_d);
+            } else {
+                const uint4* np = node ? reinterpret_cast<const uint4*>(l_w4 + cur)
+                                       : reinterpret_cast<const uint4*>(l_tris + ~cur);
+                a = np[0];
+                b = np[1];
+                c = np[2];
+                dw = np[node ? 3 : 0];
+            }
+            // Only the node branch reads dw, so the compiler would sink its
+            // load into that branch, after the triangle branch has already
+            // waited for the other three: two round trips for a wave that
+            // holds both.  These empty statements use all four records here,
+            // before the branch, so the loads are issued together, waited for
+            // once, and none is still in flight when the next step reuses
+            // their registers (which would otherwise cost a wait for every
+            // outstanding memory operation, hit stores included).
+            asm volatile("" : "+v"(dw.x), "+v"(dw.y), "+v"(dw.z), "+v"(dw.w));
+            asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w), "+v"(b.x), "+v"(b.y), "+v"(b.z),
+                         "+v"(b.w), "+v"(c.x), "+v"(c.y), "+v"(c.z), "+v"(c.w));
+            if (node) {
+                const int4 d = make_int4((int)dw.x, (int)dw.y, (int)dw.z, (int)dw.w);
+                if (STATS) nv++;
+                float k0 = __builtin_inff(), k1 = __builtin_inff(), k2 = __builtin_inff(), k3 = __builtin_inff();
+                float tn;
+                int nh = 0;
+                // child k's words, each rotated so the entry plane is the low half
+                auto box = [&](uint32_t wx, uint32_t wy, uint32_t wz, float& key) {
+                    const uint32_t x = __builtin_amdgcn_alignbit(wx, wx, rot_x);
+                    const uint32_t y = __builtin_amdgcn_alignbit(wy, wy, rot_y);
+                    const uint32_t z = __builtin_amdgcn_alignbit(wz, wz, rot_z);
+                    if (slab_ordered(lo16(x), lo16(y), lo16(z), hi16(x), hi16(y), hi16(z), cq, inv, tmin_rel,
+                                     best_rel, tn)) {
+                        key = tn;
+                        nh++;
+                    }
+                };
+                if (d.x != kNoChild) box(a.x, a.y, a.z, k0);
+                if (d.y != kNoChild) box(a.w, b.x, b.y, k1);
+                if (d.z != kNoChild) box(b.z, b.w, c.x, k2);
+                if (d.w != kNoChild) box(c.y, c.z, c.w, k3);
+                if (nh > 0) {
+                    int32_t v0 = d.x, v1 = d.y, v2 = d.z, v3 = d.w;
+                    // sort the four (entry distance, child) pairs; misses carry +inf
+                    cswap(k0, v0, k1, v1);
+                    cswap(k2, v2, k3, v3);
+                    cswap(k0, v0, k2, v2);
+                    cswap(k1, v1, k3, v3);
+                    cswap(k1, v1, k2, v2);
+                    // farthest pushed first so the nearest deferred child pops first
+                    if (nh > 3) push(v3);
+                    if (nh > 2) push(v2);
+                    if (nh > 1) push(v1);
+                    cur = v0;
+                    next = true;
+                }
+            } else {
+                const int32_t slot = ~cur;
+                const PTri tr{make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
+                                          __uint_as_float(a.w)),
+                              make_float4(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z),
+                                          __uint_as_float(b.w)),
+                              make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z),
+                                          __uint_as_float(c.w))};
+                if (STATS) nt++;
+                float t, u, v;
+                // a best handed over from a later piece yields to an equal t of this one
+                float cap = best_t;
+                if (SH && best_lo > plo) cap = __uint_as_float(__float_as_uint(best_t) + 1u);
+                if (tri_test(tr, r.o, r.d, r.tmin, cap, t, u, v)) {
+                    best_t = t; bu = u; bv = v; best = slot;
+                    best_rel = t - t_shift;
+                    if (SH) best_lo = plo;
+                }
+            }
+        } else if (sp >= 0) {
+            if (cur >= 0) {
+                bool h0, h1;
+                float tn0, tn1;
+                int32_t c0, c1;
+                if (LAYOUT == kLayoutQ16) {
+                    const uint4* np = reinterpret_cast<const uint4*>(sc.qnodes + cur);
+                    uint4 a = np[0], b = np[1];
+                    h0 = slab(lo16(a.x), hi16(a.x), lo16(a.y), hi16(a.y), lo16(a.z), hi16(a.z), bo, inv,
+                              tmin_rel, best_rel, tn0);
+                    h1 = slab(lo16(a.w), hi16(a.w), lo16(b.x), hi16(b.x), lo16(b.y), hi16(b.y), bo, inv,
+                              tmin_rel, best_rel, tn1);
+                    c0 = (int32_t)b.z;
+                    c1 = (int32_t)b.w;
+                } else {
+                    const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
+                    float4 a = np[0], b = np[1], c = np[2];
+                    int4 dd = reinterpret_cast<const int4*>(np)[3];
+                    h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, bo, inv, tmin_rel, best_rel, tn0);
+                    h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, bo, inv, tmin_rel, best_rel, tn1);
+                    c0 = dd.x;
+                    c1 = dd.y;
+                }
+                if (STATS) nv++;
+                if (h0 && h1) {
+                    bool swap = tn1 < tn0;
+                    push(swap ? c0 : c1);
+                    cur = swap ? c1 : c0;
+                    next = true;
+                } else if (h0 || h1) {
+                    cur = h0 ? c0 : c1;
+                    next = true;
+                }
+            } else {
+                const int32_t slot = ~cur;
+                const float4* tp = reinterpret_cast<const float4*>(l_tris + slot);
+                PTri tr{tp[0], tp[1], tp[2]};
+                if (STATS) nt++;
+                float t, u, v;
+                if (tri_test(tr, r.o, r.d, r.tmin, best_t, t, u, v)) {
+                    best_t = t; bu = u; bv = v; best = slot;
+                    best_rel = t - t_shift;
+                }
+            }
+        }
+        if (!next) {
+            if (sp > (SH ? bot : 0)) {
+                cur = pop();
+            } else if (SH && owner >= 0) {
+                handoff = true;             // a helper's piece is done
+                busy = false;
+            } else if (SH && pending != 0) {
+                waiting = true;             // the owner's piece is done
+                busy = false;
+            } else {
+                // ---- ray finished: the result (spheres follow in their own pass)
+                const int32_t id = best >= 0 ? l_triid[best] : -1;
+                if (id >= 0) store_hit_stream(l_hits, ray_i, id, best_t, bu, bv);
+                else store_hit_stream(l_hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
+                if (STATS) { io.node_visits[ray_i] = nv; io.tri_tests[ray_i] = nt; }
+                busy = false;
+            }
+        }
     };
 
     for (;;) {
@@ -365,6 +581,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     best_rel = r.tmax - t_shift;
                     best_t = r.tmax; bu = 0.0f; bv = 0.0f; best = -1;
                     nv = 0; nt = 0; sp = 0;
+                    bot = 0; plo = 0; pk = 31; owner = -1; pending = 0; best_lo = 0;
                     cur = root;
                     busy = true;
                     // an empty tree, or a ray that misses the frame's box
@@ -374,156 +591,106 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                 }
             }
         }
+        // ---- drain (no rays left to fetch): idle lanes take work from busy ones
+        if constexpr (STEAL) {
+            if (drained) {
+                // Idle lanes take the bottom stack entries of busy lanes.  The
+                // donors' entries were written by other lanes of this wave in
+                // earlier steps: keep every LDS access in order.
+                __builtin_amdgcn_wave_barrier();
+                const uint64_t idle_m = __ballot(!busy && !waiting);
+                const uint64_t donor_m = __ballot(busy && pk > 0 && sp - bot >= 1);
+                if ((uint32_t)__popcll(idle_m) >= io.steal && donor_m != 0) {
+                    const uint32_t k = min((uint32_t)__popcll(idle_m), (uint32_t)__popcll(donor_m));
+                    const uint32_t rank_idle = (uint32_t)__popcll(idle_m & lanes_below);
+                    const bool thief = ((idle_m >> lane) & 1ull) && rank_idle < k;
+                    const bool donor = ((donor_m >> lane) & 1ull) && (uint32_t)__popcll(donor_m & lanes_below) < k;
+                    // the i-th idle lane (in lane order) is paired with the
+                    // i-th donor; every lane takes part in the transfers
+                    const uint32_t src = thief ? nth_set_bit(donor_m, rank_idle) : lane;
+                    const float ox = shfl_f(r.o.x, src), oy = shfl_f(r.o.y, src), oz = shfl_f(r.o.z, src);
+                    const float dx = shfl_f(r.d.x, src), dy = shfl_f(r.d.y, src), dz = shfl_f(r.d.z, src);
+                    const float tmn = shfl_f(r.tmin, src);
+                    const float ix = shfl_f(inv.x, src), iy = shfl_f(inv.y, src), iz = shfl_f(inv.z, src);
+                    const float cx = shfl_f(cq.x, src), cy = shfl_f(cq.y, src), cz = shfl_f(cq.z, src);
+                    const float tsh = shfl_f(t_shift, src), tminr = shfl_f(tmin_rel, src);
+                    const float bestr = shfl_f(best_rel, src), bestt = shfl_f(best_t, src);
+                    const int v_bot = __shfl(bot, (int)src);
+                    const uint32_t v_plo = (uint32_t)__shfl((int)plo, (int)src);
+                    const uint32_t v_pk = (uint32_t)__shfl((int)pk, (int)src);
+                    const int32_t v_owner = __shfl(owner < 0 ? (int32_t)lane : owner, (int)src);
+                    if (thief) {
+                        // the donor's bottom entry: v_bot < donor's sp <= STACK
+                        cur = stk[v_bot * BLOCK + (int)(tid - lane + src)];
+                        r.o = f3{ox, oy, oz};
+                        r.d = f3{dx, dy, dz};
+                        r.tmin = tmn;
+                        inv = f3{ix, iy, iz};
+                        cq = f3{cx, cy, cz};
+                        rot_x = (__float_as_uint(ix) >> 27) & 16u;
+                        rot_y = (__float_as_uint(iy) >> 27) & 16u;
+                        rot_z = (__float_as_uint(iz) >> 27) & 16u;
+                        t_shift = tsh;
+                        tmin_rel = tminr;
+                        best_rel = bestr;
+                        best_t = bestt;
+                        bu = 0.0f; bv = 0.0f; best = -1;
+                        sp = 0; bot = 0;
+                        pk = v_pk - 1u;
+                        plo = v_plo + (1u << pk);
+                        best_lo = plo;
+                        owner = v_owner;
+                        pending = 0;
+                        busy = true;
+                    }
+                    if (donor) {
+                        bot++;
+                        pk--;
+                    }
+                    // owners count the helpers they gained
+                    uint64_t tm = __ballot(thief);
+                    while (tm != 0) {
+                        const int j = __builtin_ctzll(tm);
+                        tm &= tm - 1ull;
+                        if ((int32_t)lane == __builtin_amdgcn_readlane(owner, j)) pending++;
+                    }
+                }
+            }
+        }
+        // (a waiting owner implies a busy helper, so no busy lane means none)
         if (__ballot(busy) == 0) {
             if (drained) break;
             continue;
         }
-        // ---- one traversal step per busy lane
-        if (busy) {
-            bool next = false;          // cur holds the next record to read
-            if (sp >= 0 && kW4) {
-                // One memory round trip per step whatever the lanes' mix of
-                // nodes and triangles: every lane issues its three 16-byte
-                // loads (node words 0-2 or the triangle record) and a fourth
-                // (node refs, or the triangle's first word again, so it never
-                // reads past the array) before any lane branches.
-                const bool node = cur >= 0;
-                uint4 a, b, c, dw;
-                if constexpr (LAYOUT == kLayoutW4Joined) {
-                    // one allocation: 32-bit byte offsets from its base, so
-                    // the loads take the base from scalar registers and no
-                    // lane computes a 64-bit address
-                    const char* base = reinterpret_cast<const char*>(sc.w4nodes);
-                    const uint32_t slot = (uint32_t)(~cur);
-                    const uint32_t off_node = (uint32_t)cur << 6;
-                    const uint32_t off_tri = sc.tri_off + (slot << 5) + (slot << 4);
-                    const uint32_t off = node ? off_node : off_tri;
-                    const uint32_t off_d = node ? off_node + 48u : off_tri;
-                    a = *reinterpret_cast<const uint4*>(base + off);
-                    b = *reinterpret_cast<const uint4*>(base + off + 16u);
-                    c = *reinterpret_cast<const uint4*>(base + off + 32u);
-                    dw = *reinterpret_cast<const uint4*>(base + off_d);
-                } else {
-                    const uint4* np = node ? reinterpret_cast<const uint4*>(l_w4 + cur)
-                                           : reinterpret_cast<const uint4*>(l_tris + ~cur);
-                    a = np[0];
-                    b = np[1];
-                    c = np[2];
-                    dw = np[node ? 3 : 0];
-                }
-                // Only the node branch reads dw, so the compiler would sink its
-                // load into that branch, after the triangle branch has already
-                // waited for the other three: two round trips for a wave that
-                // holds both.  This empty statement uses dw here, before the
-                // branch, so the four loads are issued together and waited for
-                // once.
-                asm volatile("" : "+v"(dw.x), "+v"(dw.y), "+v"(dw.z), "+v"(dw.w));
-                if (node) {
-                    const int4 d = make_int4((int)dw.x, (int)dw.y, (int)dw.z, (int)dw.w);
-                    if (STATS) nv++;
-                    float k0 = __builtin_inff(), k1 = __builtin_inff(), k2 = __builtin_inff(), k3 = __builtin_inff();
-                    float tn;
-                    int nh = 0;
-                    // child k's words, each rotated so the entry plane is the low half
-                    auto box = [&](uint32_t wx, uint32_t wy, uint32_t wz, float& key) {
-                        const uint32_t x = __builtin_amdgcn_alignbit(wx, wx, rot_x);
-                        const uint32_t y = __builtin_amdgcn_alignbit(wy, wy, rot_y);
-                        const uint32_t z = __builtin_amdgcn_alignbit(wz, wz, rot_z);
-                        if (slab_ordered(lo16(x), lo16(y), lo16(z), hi16(x), hi16(y), hi16(z), cq, inv, tmin_rel,
-                                         best_rel, tn)) {
-                            key = tn;
-                            nh++;
+        step(std::integral_constant<bool, STEAL>{});
+        if constexpr (STEAL) {
+            if (drained) {
+                // ---- helpers that finished hand their result to the owner
+                uint64_t fin = __ballot(handoff);
+                while (fin != 0) {
+                    const int j = __builtin_ctzll(fin);
+                    fin &= fin - 1ull;
+                    const int32_t o = __builtin_amdgcn_readlane(owner, j);
+                    const int32_t hb = __builtin_amdgcn_readlane(best, j);
+                    const float ht = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(best_t), j));
+                    const float hu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bu), j));
+                    const float hv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bv), j));
+                    const uint32_t hlo = (uint32_t)__builtin_amdgcn_readlane((int)plo, j);
+                    if ((int32_t)lane == o) {
+                        pending--;
+                        if (hb >= 0 && (ht < best_t || (ht == best_t && hlo < best_lo))) {
+                            best = hb; best_t = ht; bu = hu; bv = hv; best_lo = hlo;
+                            best_rel = ht - t_shift;
                         }
-                    };
-                    if (d.x != kNoChild) box(a.x, a.y, a.z, k0);
-                    if (d.y != kNoChild) box(a.w, b.x, b.y, k1);
-                    if (d.z != kNoChild) box(b.z, b.w, c.x, k2);
-                    if (d.w != kNoChild) box(c.y, c.z, c.w, k3);
-                    if (nh > 0) {
-                        int32_t v0 = d.x, v1 = d.y, v2 = d.z, v3 = d.w;
-                        // sort the four (entry distance, child) pairs; misses carry +inf
-                        cswap(k0, v0, k1, v1);
-                        cswap(k2, v2, k3, v3);
-                        cswap(k0, v0, k2, v2);
-                        cswap(k1, v1, k3, v3);
-                        cswap(k1, v1, k2, v2);
-                        // farthest pushed first so the nearest deferred child pops first
-                        if (nh > 3) push(v3);
-                        if (nh > 2) push(v2);
-                        if (nh > 1) push(v1);
-                        cur = v0;
-                        next = true;
-                    }
-                } else {
-                    const int32_t slot = ~cur;
-                    const PTri tr{make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
-                                              __uint_as_float(a.w)),
-                                  make_float4(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z),
-                                              __uint_as_float(b.w)),
-                                  make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z),
-                                              __uint_as_float(c.w))};
-                    if (STATS) nt++;
-                    float t, u, v;
-                    if (tri_test(tr, r.o, r.d, r.tmin, best_t, t, u, v)) {
-                        best_t = t; bu = u; bv = v; best = slot;
-                        best_rel = t - t_shift;
                     }
                 }
-            } else if (sp >= 0) {
-                if (cur >= 0) {
-                    bool h0, h1;
-                    float tn0, tn1;
-                    int32_t c0, c1;
-                    if (LAYOUT == kLayoutQ16) {
-                        const uint4* np = reinterpret_cast<const uint4*>(sc.qnodes + cur);
-                        uint4 a = np[0], b = np[1];
-                        h0 = slab(lo16(a.x), hi16(a.x), lo16(a.y), hi16(a.y), lo16(a.z), hi16(a.z), bo, inv,
-                                  tmin_rel, best_rel, tn0);
-                        h1 = slab(lo16(a.w), hi16(a.w), lo16(b.x), hi16(b.x), lo16(b.y), hi16(b.y), bo, inv,
-                                  tmin_rel, best_rel, tn1);
-                        c0 = (int32_t)b.z;
-                        c1 = (int32_t)b.w;
-                    } else {
-                        const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
-                        float4 a = np[0], b = np[1], c = np[2];
-                        int4 dd = reinterpret_cast<const int4*>(np)[3];
-                        h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, bo, inv, tmin_rel, best_rel, tn0);
-                        h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, bo, inv, tmin_rel, best_rel, tn1);
-                        c0 = dd.x;
-                        c1 = dd.y;
-                    }
-                    if (STATS) nv++;
-                    if (h0 && h1) {
-                        bool swap = tn1 < tn0;
-                        push(swap ? c0 : c1);
-                        cur = swap ? c1 : c0;
-                        next = true;
-                    } else if (h0 || h1) {
-                        cur = h0 ? c0 : c1;
-                        next = true;
-                    }
-                } else {
-                    const int32_t slot = ~cur;
-                    const float4* tp = reinterpret_cast<const float4*>(l_tris + slot);
-                    PTri tr{tp[0], tp[1], tp[2]};
-                    if (STATS) nt++;
-                    float t, u, v;
-                    if (tri_test(tr, r.o, r.d, r.tmin, best_t, t, u, v)) {
-                        best_t = t; bu = u; bv = v; best = slot;
-                        best_rel = t - t_shift;
-                    }
-                }
-            }
-            if (!next) {
-                if (sp > 0) {
-                    cur = pop();
-                } else {
-                    // ---- ray finished: the result (spheres follow in their own pass)
-                    const int32_t id = best >= 0 ? l_triid[best] : -1;
-                    if (id >= 0) store_hit_stream(l_hits, ray_i, id, best_t, bu, bv);
-                    else store_hit_stream(l_hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
-                    if (STATS) { io.node_visits[ray_i] = nv; io.tri_tests[ray_i] = nt; }
-                    busy = false;
+                handoff = false;
+                // an owner whose helpers are all done writes its hit in the
+                // next step (sp = -1 finishes a ray at once)
+                if (waiting && pending == 0) {
+                    waiting = false;
+                    busy = true;
+                    sp = -1;
                 }
             }
         }

@@ -21,6 +21,7 @@ RT_MAT_LIGHT, RT_MAT_LAMBERTIAN, RT_MAT_METAL, RT_MAT_GLASS = range(4)
 SPHERES_OFF, SPHERES_NEAREST, SPHERES_SCENE = range(3)
 TREE_GIVEN, TREE_DEVICE = range(2)
 ORDER_AUTO, ORDER_GIVEN, ORDER_SORTED = range(3)
+STEAL_NEVER = 65
 TRAVERSE_AUTO, TRAVERSE_PACKED, TRAVERSE_REFERENCE = range(3)
 LAYOUT_AUTO, LAYOUT_F32, LAYOUT_Q16, LAYOUT_W4 = range(4)
 
@@ -74,7 +75,8 @@ class RenderParams(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("node_layout", ctypes.c_uint32), ("refill", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("stack_lds", ctypes.c_uint32),
-                ("tree", ctypes.c_uint32), ("ray_order", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("tree", ctypes.c_uint32), ("ray_order", ctypes.c_uint32), ("steal", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class Ubo(ctypes.Structure):
@@ -470,8 +472,8 @@ class Context:
     def check(self, rc, what=""):
         _check(rc, self._h, what)
 
-    def set_tuning(self, node_layout=0, refill=0, blocks_per_cu=0, stack_lds=0, tree=0, ray_order=0):
-        t = Tuning(node_layout, refill, blocks_per_cu, stack_lds, tree, ray_order)
+    def set_tuning(self, node_layout=0, refill=0, blocks_per_cu=0, stack_lds=0, tree=0, ray_order=0, steal=0):
+        t = Tuning(node_layout, refill, blocks_per_cu, stack_lds, tree, ray_order, steal)
         self.check(lib().rt_tuning_set(self._h, ctypes.byref(t)), "rt_tuning_set")
 
     def get_tuning(self) -> Tuning:
