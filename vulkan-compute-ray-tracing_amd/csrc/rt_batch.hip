@@ -184,7 +184,7 @@ int launch_batch_kernel(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
     if (!blocks) return RT_OK;
     int32_t* spill = RING ? b->d_spill : nullptr;
     TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, b->d_cursor, b->total, b->cfg.refill, nullptr,
-               b->total >= (1u << 16) ? kSegments : 1u};
+               b->total >= (1u << 16) ? kSegments : 1u, drain_steal(ctx)};
     BatchIo bio{b->d_table, b->d_offsets, b->d_chunk_item, b->cfg.regroup ? b->d_perm : nullptr};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), 0, s, io, bio, spill);
     RT_HIP(ctx, hipGetLastError());
@@ -292,7 +292,7 @@ int launch_dyn_t(rt_ctx* ctx, const BatchIo& bio, uint32_t* cursor, uint32_t tot
     if (need < blocks) blocks = need;
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, cursor, 0u, kCursorBlock);
     TraceIo io{nullptr, nullptr, nullptr, nullptr, nullptr, cursor, total, refill, nullptr,
-               total >= (1u << 16) ? kSegments : 1u};
+               total >= (1u << 16) ? kSegments : 1u, drain_steal(ctx)};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, io, bio, (int32_t*)nullptr);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;

@@ -255,6 +255,9 @@ int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
 // for it (or, by default, for large batches on large scenes); *order stays
 // null for index order.
 int ray_order(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, const uint32_t** order);
+// Idle lanes that make a drained wave share its busy lanes' work
+// (rt_tuning.steal, default 32).
+uint32_t drain_steal(const rt_ctx* ctx);
 int resolved_layout(const rt_ctx* ctx);   // RT_LAYOUT_* actually used
 int launch_trace_reference(rt_ctx* ctx, const TraceArgs& a, uint32_t max_stack, hipStream_t s);
 // Spheres after a traversal (rt_sphere.hip): no-op unless a.spheres is set

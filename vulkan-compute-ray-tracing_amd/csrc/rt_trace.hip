@@ -216,12 +216,8 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
     if (refill > 64u) refill = 64u;
     TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->frame,
                   LAYOUT == kLayoutW4 || LAYOUT == kLayoutW4Joined ? sc->root4 : sc->root_ref, sc->tri_off};
-    // idle lanes before a drained wave hands subtrees to them: a hand-over
-    // step costs about a traversal step, so it waits for many takers.
-    // Measured on the 4M-triangle query (ms per launch): 8 6.42, 16 6.28,
-    // 24 6.22, 32 6.22, 40 6.31, 48 6.44, never 7.63.
-    const uint32_t steal = ctx->tuning.steal ? ctx->tuning.steal : 32u;
-    TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill, a.order, segs, steal};
+    TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill, a.order, segs,
+               drain_steal(ctx)};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, ts, io, spill);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;
@@ -244,6 +240,12 @@ int launch_w4(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
 }
 
 }  // namespace
+
+// Idle lanes before a drained wave hands subtrees to them: a hand-over step
+// costs about a traversal step, so it waits for many takers.  Measured on the
+// 4M-triangle query (ms per launch): 8 6.42, 16 6.28, 24 6.22, 32 6.22,
+// 40 6.31, 48 6.44, never 7.63.
+uint32_t drain_steal(const rt_ctx* ctx) { return ctx->tuning.steal ? ctx->tuning.steal : 32u; }
 
 // The sorted order of a query's rays in ctx's order workspace, or null when
 // the batch keeps index order.

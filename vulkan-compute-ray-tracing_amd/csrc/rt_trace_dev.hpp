@@ -256,7 +256,7 @@ __device__ __forceinline__ float shfl_f(float x, uint32_t src) {
 // push/pop beyond RING.  RING = 0 keeps the whole stack (STACK entries) in LDS.
 //
 // Work sharing in the drain (4-wide layouts, whole stack in LDS, no
-// counters): once a wave's rays are used up its idle lanes would wait for the
+// counters; in batched launches the helper also takes the donor's scene): once a wave's rays are used up its idle lanes would wait for the
 // longest rays of the wave.  Instead each idle lane takes the bottom entry of
 // a busy lane's stack -- the subtree that lane would visit last -- and
 // traverses it for the ray's owner (a helper).  The ordered traversal's
@@ -274,7 +274,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                                                  int32_t* __restrict__ stk, int32_t* __restrict__ spill) {
     static_assert(!BATCHED || LAYOUT == kLayoutW4, "batched launches use the 4-wide layout");
     constexpr bool kW4 = LAYOUT == kLayoutW4 || LAYOUT == kLayoutW4Joined;
-    constexpr bool STEAL = kW4 && RING == 0 && !STATS && !BATCHED;
+    constexpr bool STEAL = kW4 && RING == 0 && !STATS;
     static_assert(RING == 0 || (RING & (RING - 1)) == 0, "ring size is a power of two");
     const uint32_t n = io.n_dev ? *io.n_dev : io.n_host;
     const uint32_t tid = threadIdx.x;
@@ -619,7 +619,25 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     const uint32_t v_plo = (uint32_t)__shfl((int)plo, (int)src);
                     const uint32_t v_pk = (uint32_t)__shfl((int)pk, (int)src);
                     const int32_t v_owner = __shfl(owner < 0 ? (int32_t)lane : owner, (int)src);
+                    // batched lanes each carry their item's scene
+                    auto shfl_ptr = [&](const void* q) -> uint64_t {
+                        const uint64_t v = (uint64_t)q;
+                        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)src);
+                        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), (int)src);
+                        return ((uint64_t)hi << 32) | lo;
+                    };
+                    uint64_t v_w4 = 0, v_tris = 0, v_triid = 0;
+                    if constexpr (BATCHED) {
+                        v_w4 = shfl_ptr(l_w4);
+                        v_tris = shfl_ptr(l_tris);
+                        v_triid = shfl_ptr(l_triid);
+                    }
                     if (thief) {
+                        if constexpr (BATCHED) {
+                            l_w4 = reinterpret_cast<const PNode4*>(v_w4);
+                            l_tris = reinterpret_cast<const PTri*>(v_tris);
+                            l_triid = reinterpret_cast<const int32_t*>(v_triid);
+                        }
                         // the donor's bottom entry: v_bot < donor's sp <= STACK
                         cur = stk[v_bot * BLOCK + (int)(tid - lane + src)];
                         r.o = f3{ox, oy, oz};
