@@ -111,3 +111,37 @@ def test_batch_rejects_scene_without_packed_path(rt, gpu_ctx):
         rt.Batch(gpu_ctx, [(scene, rays, hits)])
     assert e.value.status == rt.RT_ERR_UNSUPPORTED
     scene.release()
+
+
+@pytest.mark.parametrize("steal", [0, 1])
+def test_batch_drain_sharing_on_ties(rt, gpu_ctx, steal):
+    """Drained waves of a batched launch hand subtrees (with their item's
+    scene) to idle lanes.  On the tie grid, where every ray ends on an edge
+    or a vertex shared at equal t, and on a clustered soup in the same
+    launch, every item's hits equal the counting kernel's (which never
+    shares) byte for byte."""
+    from test_gpu_query import tie_grid, tie_rays
+    tris = tie_grid(rt)
+    grid = gpu_ctx.upload(rt.HostScene(tris, np.zeros(1, rt.MAT_DT), rt.bvh_build(tris, seed=1)))
+    soup = gpu_ctx.upload(rt.scene_clusters(60000, 31))
+    items, refs = [], []
+    for k, (scene, rays_np) in enumerate([(grid, tie_rays(rt, 50000, seed=41)),
+                                          (soup, rt.rays_box(30000, 42, 4.0)),
+                                          (grid, tie_rays(rt, 7777, seed=43))]):
+        rays = rt.rays_to_device(rays_np)
+        ref, _, _ = gpu_ctx.closest_hit_stats(scene, rays)
+        items.append((scene, rays, torch.full((len(rays_np), 4), -7.0, device="cuda")))
+        refs.append(ref)
+    torch.cuda.synchronize()
+    refs = [rt.hits_from_device(r) for r in refs]
+    print(f"case: batched drain sharing on the tie grid, steal {steal}", flush=True)
+    gpu_ctx.set_tuning(steal=steal)
+    try:
+        b = rt.Batch(gpu_ctx, items, rt.batch_config(stack_lds=0))
+        b.launch()
+        check(rt, items, refs)
+        b.destroy()
+    finally:
+        gpu_ctx.set_tuning()
+    grid.release()
+    soup.release()
