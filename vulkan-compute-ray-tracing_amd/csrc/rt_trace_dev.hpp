@@ -598,6 +598,14 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                 // donors' entries were written by other lanes of this wave in
                 // earlier steps: keep every LDS access in order.
                 __builtin_amdgcn_wave_barrier();
+                // Helpers cull against their owner's closest hit as soon as it
+                // improves (the subtrees they hold are the ones a single lane
+                // would visit last, often culled by then).  Culling only: the
+                // closest hit and its ties lie at or before the owner's t, so
+                // no box that holds them is dropped, and acceptance still
+                // compares with the helper's own best.
+                const float owner_rel = shfl_f(best_rel, owner < 0 ? lane : (uint32_t)owner);
+                if (busy && owner >= 0) best_rel = fminf(best_rel, owner_rel);
                 const uint64_t idle_m = __ballot(!busy && !waiting);
                 const uint64_t donor_m = __ballot(busy && pk > 0 && sp - bot >= 1);
                 if ((uint32_t)__popcll(idle_m) >= io.steal && donor_m != 0) {
