@@ -226,7 +226,7 @@ int launch_slot(rt_server* S, Slot& sl) {
         BatchIo bio{sl.d_table, sl.d_offsets, sl.d_chunk, nullptr};
         // refill threshold as for a plain query: early refills when most rays
         // trace a large scene (long traversals), late ones otherwise
-        const uint32_t refill = 2ull * long_rays >= off ? 16u : 48u;
+        const uint32_t refill = 2ull * long_rays >= off ? 24u : 48u;
         int rc = launch_batch_dynamic(sl.ctx, bio, sl.d_cursor, off, need, refill, st);
         if (rc) return rc;
         for (Request* r : sl.reqs) {

@@ -209,10 +209,12 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
     // idle).  The loop is VALU-bound, so idle lanes waste issue slots: a
     // large scene's long traversals want lanes refilled early, while a small
     // scene's short ones finish so fast that each refill's fetches are better
-    // shared by more rays.  Measured: 16 is best from 2^17 triangles up
-    // (-5..8% time against 32), 48 below (-1..4%).
+    // shared by more rays.  Measured: 24 is best from 2^17 triangles up
+    // (4M-triangle query 6.35 ms against 6.72 at 16 and 6.46 at 32, with the
+    // drain's work sharing taking over the lanes idle at the end), 48 below
+    // (-1..4% against 32).
     uint32_t refill = ctx->tuning.refill ? ctx->tuning.refill
-                                         : (sc->info.num_triangles >= (1u << 17) ? 16u : 48u);
+                                         : (sc->info.num_triangles >= (1u << 17) ? 24u : 48u);
     if (refill > 64u) refill = 64u;
     TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->frame,
                   LAYOUT == kLayoutW4 || LAYOUT == kLayoutW4Joined ? sc->root4 : sc->root_ref, sc->tri_off};
