@@ -526,7 +526,11 @@ def main():
                      "kernel": "k_trace_packed", "kernel_ms": round(kernel_ms, 4),
                      "call_ms": round(call_ms, 4), "order_ms": round(order_ms, 4),
                      "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "stream_copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
+                     "stream_copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
+                     # measured HBM bytes (the traffic figure) over the same kernel time: the
+                     # algorithmic bytes count every record fetch, about half of which hit L2
+                     "hbm_gbs": round(traffic / (kernel_ms / 1e3) / 1e9, 1) if traffic else None,
+                     "hbm_frac": round(traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None},
         "visits": {"node_records_per_ray": round(v_node / n, 2), "tri_records_per_ray": round(v_tri / n, 3)},
     }
 
