@@ -493,7 +493,7 @@ extern "C" int rt_batch_autotune(rt_ctx* ctx, rt_batch* b, uint32_t trials, cons
     float cur_ms = 0.0f;
     int rc = time_cfg(cur, cur_ms);
     const std::vector<uint32_t> blocks = {256, 128}, rings = {0, 32, 16, 8}, regroups = {0, 1},
-                                refills = {8, 16, 32, 48, 64}, bpcs = {0, 2, 3, 4};
+                                refills = {8, 16, 24, 32, 48, 64}, bpcs = {0, 2, 3, 4};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
         for (int p = 0; p < 5 && !rc; ++p) {
             const std::vector<uint32_t>& vals = p == 0 ? blocks : p == 1 ? rings : p == 2 ? regroups : p == 3 ? refills : bpcs;

@@ -193,6 +193,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: run vulkan-compute-ray-tracing_amd/build.py")
+    # torch ships its own HIP runtime.  With librtamd.so loaded and a context
+    # created first, torch's later CUDA initialisation found no GPU; with
+    # torch loaded first both share one runtime.  So torch, when installed,
+    # is always imported before the library is loaded.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P, u32, i32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64
     sig = {
