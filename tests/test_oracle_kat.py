@@ -21,8 +21,8 @@ def tri(v0, v1, v2, mat=0):
 
 
 # --------------------------------------------------------------------- RNG
-# random.glsl:4-22.  Values derived from the integer formula in the survey
-# session (a separate computation), for (x, y, currentSample).
+# random.glsl:4-22.  Values derived from the integer formula by a separate
+# computation (not the oracle), for (x, y, currentSample).
 RNG_KAT = {
     (0, 0, 0): ([0x108EF29B, 0x00033628, 0xDCCC2102], [0x3D847795, 0x384D8A00, 0x3F5CCC21]),
     (1, 0, 0): ([0x9F21590F, 0x44E67BC7, 0x60B37254], [0x3F1F2159, 0x3E89CCF8, 0x3EC166E5]),
