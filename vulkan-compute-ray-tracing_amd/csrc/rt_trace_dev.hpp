@@ -256,7 +256,8 @@ __device__ __forceinline__ float shfl_f(float x, uint32_t src) {
 // push/pop beyond RING.  RING = 0 keeps the whole stack (STACK entries) in LDS.
 //
 // Work sharing in the drain (4-wide layouts, whole stack in LDS, no
-// counters; in batched launches the helper also takes the donor's scene): once a wave's rays are used up its idle lanes would wait for the
+// counters; in batched launches the helper also takes the donor's scene):
+// once a wave's rays are used up, its idle lanes would wait for the
 // longest rays of the wave.  Instead each idle lane takes the bottom entry of
 // a busy lane's stack -- the subtree that lane would visit last -- and
 // traverses it for the ray's owner (a helper).  The ordered traversal's
