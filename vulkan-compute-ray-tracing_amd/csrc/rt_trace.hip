@@ -128,6 +128,7 @@ uint32_t* next_cursor(rt_ctx* ctx) {
 // Only the order changes: each ray is still traced alone and its hit lands
 // at its own index.
 constexpr uint32_t kOrderBits = 8;                 // cells per axis: 2^8
+constexpr uint32_t kQueueOrderBits = 7;            // render queues: 2^7 per axis, three radix passes
 constexpr uint32_t kOrderMinRays = 1u << 17;
 constexpr uint32_t kOrderMinTriangles = 1u << 20;
 
@@ -164,11 +165,12 @@ __global__ __launch_bounds__(256) void k_queue_keys(const rt_ray* __restrict__ r
                                                     uint32_t* __restrict__ idx) {
     const uint32_t n = *n_dev;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
-        uint32_t k = 1u << (3 * kOrderBits);
+        uint32_t k = 1u << (3 * kQueueOrderBits);
         if (i < n) {
             const float4 o = reinterpret_cast<const float4*>(rays)[2 * (size_t)i];
-            k = (spread10(order_cell(o.x, f.ox, f.rx)) << 2) | (spread10(order_cell(o.y, f.oy, f.ry)) << 1) |
-                spread10(order_cell(o.z, f.oz, f.rz));
+            constexpr uint32_t sh = kOrderBits - kQueueOrderBits;
+            k = (spread10(order_cell(o.x, f.ox, f.rx) >> sh) << 2) |
+                (spread10(order_cell(o.y, f.oy, f.ry) >> sh) << 1) | spread10(order_cell(o.z, f.oz, f.rz) >> sh);
         }
         keys[i] = k;
         idx[i] = i;
@@ -261,7 +263,9 @@ int ray_order(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, const uint32_t** o
     if (mode == RT_ORDER_AUTO && (a.n < kOrderMinRays || a.scene->info.num_triangles < kOrderMinTriangles))
         return RT_OK;
     const uint32_t n = a.n;
-    const uint32_t key_bits = 3u * kOrderBits + (a.n_dev ? 1u : 0u);
+    // a render queue's keys carry a top bit for the dead slots past its live
+    // count; 7 bits per axis keep them at 22 bits, three 8-bit radix passes
+    const uint32_t key_bits = a.n_dev ? 3u * kQueueOrderBits + 1u : 3u * kOrderBits;
     size_t sort_bytes = 0;
     RT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                           (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0u, key_bits, s));
