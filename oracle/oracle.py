@@ -10,6 +10,7 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -71,37 +72,56 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
+def _load(path: str):
+    """Loads one build of the restatement and declares its entry points."""
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    u32 = ctypes.c_uint32
+    L.orc_sizeof.restype = ctypes.c_int
+    L.orc_step_rng.restype = u32
+    L.orc_step_rng.argtypes = [u32]
+    L.orc_rng_seed.restype = u32
+    L.orc_rng_seed.argtypes = [u32, u32, u32]
+    L.orc_rng_stream.argtypes = [u32, u32, u32, u32, P, P]
+    L.orc_intersect_aabb.argtypes = [P, P, P, P, P]
+    L.orc_tri_intersect.argtypes = [P, P, P, P, P]
+    L.orc_hit_triangle.argtypes = [P, P, P, ctypes.c_float, ctypes.c_float, P]
+    L.orc_hit_sphere.argtypes = [P, P, P, ctypes.c_float, ctypes.c_float, P]
+    L.orc_closest_hit.argtypes = [P, P, u32, P, ctypes.c_int, u32, ctypes.c_int, P, P, P, ctypes.c_int]
+    L.orc_scatter.argtypes = [P, P, P, P, ctypes.c_int, u32, P, P, P]
+    L.orc_scatter.restype = ctypes.c_int
+    L.orc_accumulate.argtypes = [P, P, u32, P]
+    L.orc_render.argtypes = [P, P, P, u32, u32, u32, P, P, P, ctypes.c_int]
+    L.orc_render.restype = ctypes.c_uint64
+    L.orc_camera_rays.argtypes = [P, ctypes.c_float, u32, u32, P]
+    L.orc_morton.restype = ctypes.c_uint64
+    L.orc_morton.argtypes = [u32, u32, u32]
+    L.orc_lbvh.restype = ctypes.c_int
+    L.orc_lbvh.argtypes = [P, u32, P, P, P]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.c_void_p
-        u32 = ctypes.c_uint32
-        L.orc_sizeof.restype = ctypes.c_int
-        L.orc_step_rng.restype = u32
-        L.orc_step_rng.argtypes = [u32]
-        L.orc_rng_seed.restype = u32
-        L.orc_rng_seed.argtypes = [u32, u32, u32]
-        L.orc_rng_stream.argtypes = [u32, u32, u32, u32, P, P]
-        L.orc_intersect_aabb.argtypes = [P, P, P, P, P]
-        L.orc_tri_intersect.argtypes = [P, P, P, P, P]
-        L.orc_hit_triangle.argtypes = [P, P, P, ctypes.c_float, ctypes.c_float, P]
-        L.orc_hit_sphere.argtypes = [P, P, P, ctypes.c_float, ctypes.c_float, P]
-        L.orc_closest_hit.argtypes = [P, P, u32, P, ctypes.c_int, u32, ctypes.c_int, P, P, P, ctypes.c_int]
-        L.orc_scatter.argtypes = [P, P, P, P, ctypes.c_int, u32, P, P, P]
-        L.orc_scatter.restype = ctypes.c_int
-        L.orc_accumulate.argtypes = [P, P, u32, P]
-        L.orc_render.argtypes = [P, P, P, u32, u32, u32, P, P, P, ctypes.c_int]
-        L.orc_render.restype = ctypes.c_uint64
-        L.orc_camera_rays.argtypes = [P, ctypes.c_float, u32, u32, P]
-        L.orc_morton.restype = ctypes.c_uint64
-        L.orc_morton.argtypes = [u32, u32, u32]
-        L.orc_lbvh.restype = ctypes.c_int
-        L.orc_lbvh.argtypes = [P, u32, P, P, P]
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def using_library(path: str):
+    """Runs this module's functions on another build of the restatement (a
+    mutant compiled by a test, to show that a known-answer test detects a
+    planted mistake), then restores the committed one."""
+    global _lib
+    saved = lib()
+    _lib = _load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def _ptr(a: np.ndarray | None):

@@ -626,6 +626,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     const float bestr = shfl_f(best_rel, src), bestt = shfl_f(best_t, src);
                     const int v_bot = __shfl(bot, (int)src);
                     const uint32_t v_plo = (uint32_t)__shfl((int)plo, (int)src);
+                    const uint32_t v_best_lo = (uint32_t)__shfl((int)best_lo, (int)src);
                     const uint32_t v_pk = (uint32_t)__shfl((int)pk, (int)src);
                     const int32_t v_owner = __shfl(owner < 0 ? (int32_t)lane : owner, (int)src);
                     // batched lanes each carry their item's scene
@@ -665,7 +666,11 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                         sp = 0; bot = 0;
                         pk = v_pk - 1u;
                         plo = v_plo + (1u << pk);
-                        best_lo = plo;
+                        // the donor's best keeps the piece it came from: a
+                        // best an earlier helper handed over may come from a
+                        // piece after this one, and then an equal t found
+                        // here must replace it (the cap rule in step)
+                        best_lo = v_best_lo;
                         owner = v_owner;
                         pending = 0;
                         busy = true;
