@@ -16,6 +16,14 @@
  *   - Pointers named *_dev are device (HBM) pointers; everything else is host
  *     memory.  Streams are hipStream_t passed as void* (NULL = the context's
  *     own stream).  A context is externally synchronised.
+ *   - All device work on one context must be ordered on ONE stream: a
+ *     context's workspaces (the sorted trace order, render path state,
+ *     traversal stacks) are shared by its launches, and calls return before
+ *     their kernels finish.  Two launches of one context on different
+ *     streams may overlap on the device and overwrite each other's
+ *     workspaces.  Work that must overlap uses one context per stream (the
+ *     server gives each of its slots its own); scenes can be shared by
+ *     contexts on the same device.
  */
 #ifndef RTAMD_H
 #define RTAMD_H
@@ -39,24 +47,24 @@ enum {
     RT_ERR_NOT_FOUND = 7    /* unknown scene / request id                   */
 };
 
-/* Material type ids: definitions.glsl:37-40, GpuModels.h:122-128. */
+/* Material type ids: definitions.glsl:37-40, GpuModels.h:18-24. */
 enum { RT_MAT_LIGHT = 0, RT_MAT_LAMBERTIAN = 1, RT_MAT_METAL = 2, RT_MAT_GLASS = 3 };
 
 /* ----------------------------------------------------------------- records */
-/* GpuModel::Triangle (GpuModels.h:136-142): vec3 slots at 0/16/32, index @44. */
+/* GpuModel::Triangle (GpuModels.h:32-38): vec3 slots at 0/16/32, index @44. */
 typedef struct rt_triangle {
     float v0[3]; float _pad0;
     float v1[3]; float _pad1;
     float v2[3]; uint32_t materialIndex;
 } rt_triangle;                                   /* 48 bytes */
 
-/* GpuModel::Material (GpuModels.h:130-134): type @0, albedo @16. */
+/* GpuModel::Material (GpuModels.h:26-30): type @0, albedo @16. */
 typedef struct rt_material {
     uint32_t type; uint32_t _pad0[3];
     float albedo[3]; float _pad1;
 } rt_material;                                   /* 32 bytes */
 
-/* GpuModel::BvhNode (GpuModels.h:151-158): min @0, max @16, left @28,
+/* GpuModel::BvhNode (GpuModels.h:47-54): min @0, max @16, left @28,
  * right @32, objectIndex @36; -1 marks "none". */
 typedef struct rt_bvh_node {
     float min[3]; float _pad0;
@@ -64,12 +72,12 @@ typedef struct rt_bvh_node {
     int32_t rightNodeIndex; int32_t objectIndex; int32_t _pad1[2];
 } rt_bvh_node;                                   /* 48 bytes */
 
-/* GpuModel::Light (GpuModels.h:161-167). */
+/* GpuModel::Light (GpuModels.h:57-63). */
 typedef struct rt_light {
     uint32_t triangleIndex; float area;
 } rt_light;                                      /* 8 bytes */
 
-/* GpuModel::Sphere (GpuModels.h:144-148): (cx,cy,cz,r) @0, index @16. */
+/* GpuModel::Sphere (GpuModels.h:40-44): (cx,cy,cz,r) @0, index @16. */
 typedef struct rt_sphere {
     float s[4]; uint32_t materialIndex; uint32_t _pad[3];
 } rt_sphere;                                     /* 32 bytes */
@@ -378,7 +386,10 @@ typedef struct rt_server_config {
     uint32_t max_batch_rays;  /* rays per batched launch (0 = 1,048,576; <= 2^30) */
     uint32_t max_render_pixels; /* largest render request (0 = 1,048,576; <= 2^30) */
     uint64_t hbm_budget;      /* bytes for resident scenes (0 = no limit)       */
-    uint32_t reserved[4];
+    uint32_t deadline_ms;     /* a request still queued this long after its
+                                 submission is dropped with RT_ERR_CANCELLED
+                                 (0 = no deadline)                              */
+    uint32_t reserved[3];
 } rt_server_config;
 
 typedef struct rt_server_stats {
@@ -387,6 +398,8 @@ typedef struct rt_server_stats {
     uint64_t rays;                 /* traversals done (query rays + render rays) */
     uint64_t resident_bytes;
     double p50_ms, p99_ms, mean_ms, max_ms;   /* over completed requests        */
+    uint64_t peak_resident_bytes;  /* most scene bytes ever allocated at once    */
+    uint64_t cancelled;            /* dropped by rt_server_cancel or a deadline  */
 } rt_server_stats;
 
 typedef struct rt_server rt_server;
@@ -397,8 +410,12 @@ int rt_host_alloc(size_t bytes, void** out);
 int rt_host_free(void* p);
 int rt_server_create(rt_ctx* ctx, const rt_server_config* config, rt_server** out);
 int rt_server_destroy(rt_server* server);
-/* Uploads a scene and makes it resident, evicting least recently used idle
- * scenes while the budget is exceeded; RT_ERR_NOMEM when it cannot fit. */
+/* Makes a scene resident.  Admission comes before any device allocation:
+ * the scene is validated and packed on the host, which fixes its device
+ * bytes; least recently used idle scenes are then evicted (and freed) while
+ * the budget, or the device's free memory, cannot hold it, and only then is
+ * it uploaded -- resident bytes never exceed the budget, not even during a
+ * swap.  RT_ERR_NOMEM when it cannot fit. */
 int rt_server_scene_add(rt_server* server, const rt_scene_desc* desc, uint64_t* scene_id);
 int rt_server_scene_remove(rt_server* server, uint64_t scene_id);
 /* Writes up to `cap` resident scene ids; *count receives the total. */
@@ -410,8 +427,16 @@ int rt_server_submit_query(rt_server* server, uint64_t scene_id, const rt_ray* r
 int rt_server_submit_render(rt_server* server, uint64_t scene_id, const rt_ubo* ubo,
                             const rt_render_params* params, uint32_t w, uint32_t h, uint8_t* target,
                             uint64_t* request_id);
-/* Waits for a request (timeout_ms < 0: forever); *status gets its result. */
+/* Waits for a request (timeout_ms < 0: forever); *status gets its result:
+ * RT_OK, RT_ERR_CANCELLED (cancelled, or dropped at its deadline), or the
+ * launch's error. */
 int rt_server_wait(rt_server* server, uint64_t request_id, int timeout_ms, int* status);
+/* Cancels a request that is still queued: it completes at once with
+ * RT_ERR_CANCELLED and never runs.  RT_ERR_BUSY when it is already on the
+ * device (it then completes normally), RT_ERR_NOT_FOUND when it is unknown
+ * or finished.  The reference has no requests to cancel: a failure there
+ * ends the process (main.cpp:449-457). */
+int rt_server_cancel(rt_server* server, uint64_t request_id);
 /* Waits until every submitted request has completed. */
 int rt_server_drain(rt_server* server);
 int rt_server_stats_get(rt_server* server, rt_server_stats* stats);

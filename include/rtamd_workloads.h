@@ -95,6 +95,38 @@ typedef struct rtw_serve_result {
 
 int rtw_serve_poisson(rt_server* server, const rtw_serve_params* params, rtw_serve_result* result);
 
+/* The same run, keeping some requests' results for checking.  A query's
+ * rays are the first `count` of the run's ray pool, rtw_rays_box(query_max,
+ * seed ^ 0x5eed, 4.0); a render is eye (0, 0, 10) (camPos (0, 10, 0)),
+ * vfov 60, render_bounces bounces, currentSample 0.  Arrival k is kept when
+ * k % every == 0 while entries are left; its result goes to the entry's own
+ * buffers (hits: query_max records, image: render_w * render_h * 4 bytes,
+ * both the caller's), not to the run's shared ring.  The k-th scene the
+ * swapper adds is rtw_scene_clusters(swap_triangles, seed + 1000 + k) and
+ * its server id is written to swap_ids[k]. */
+typedef struct rtw_serve_sample {
+    rt_hit* hits;                /* in: caller's buffer for a kept query      */
+    uint8_t* image;              /* in: caller's buffer for a kept render     */
+    uint64_t scene_id;           /* out: the server scene it ran on           */
+    uint64_t arrival;            /* out: k                                    */
+    uint32_t kind;               /* out: 0 = query, 1 = render                */
+    uint32_t count;              /* out: query rays                           */
+} rtw_serve_sample;
+
+typedef struct rtw_serve_sampling {
+    uint32_t every;              /* keep arrivals k with k % every == 0 (>= 1)  */
+    uint32_t capacity;           /* entries in samples                         */
+    rtw_serve_sample* samples;
+    uint32_t taken;              /* out: entries filled                        */
+    uint32_t swaps_capacity;     /* entries in swap_ids                        */
+    uint64_t* swap_ids;
+    uint32_t swaps_taken;        /* out                                        */
+    uint32_t reserved;
+} rtw_serve_sampling;
+
+int rtw_serve_poisson_sampled(rt_server* server, const rtw_serve_params* params, rtw_serve_sampling* sampling,
+                              rtw_serve_result* result);
+
 /* Log-uniform integer in [lo, hi]: floor(exp(U * ln(hi/lo)) * lo), U from
  * the seed's first draw (used for scene and batch sizes). */
 uint32_t rtw_log_uniform(uint64_t seed, uint32_t lo, uint32_t hi);

@@ -105,6 +105,7 @@ extern "C" int rt_tuning_set(rt_ctx* ctx, const rt_tuning* t) {
     if (t->stack_lds != 0 && t->stack_lds != 8 && t->stack_lds != 16 && t->stack_lds != 32)
         return set_error(ctx, RT_ERR_INVALID, "rt_tuning_set: stack_lds must be 0, 8, 16 or 32");
     ctx->tuning = *t;
+    ctx->tuning_gen++;          // batches captured under the old tuning re-capture
     return RT_OK;
 }
 

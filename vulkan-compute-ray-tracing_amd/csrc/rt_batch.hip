@@ -154,6 +154,7 @@ struct rt_batch {
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t tuning_gen = 0;           // the context's tuning the launch was set up under
 };
 
 namespace {
@@ -241,6 +242,7 @@ void drop_graph(rt_batch* b) {
 // into a graph when the configuration asks for one.
 int capture(rt_ctx* ctx, rt_batch* b) {
     drop_graph(b);
+    b->tuning_gen = ctx->tuning_gen;
     b->dry = true;
     int rc = launch_sequence(ctx, b, nullptr);
     b->dry = false;
@@ -411,6 +413,12 @@ extern "C" int rt_batch_launch(rt_ctx* ctx, rt_batch* b, void* stream) {
     if (b->total == 0) return RT_OK;
     RT_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)stream;
+    // the context's tuning (the drain's steal threshold) changed since the
+    // launch was set up: set it up again, so a replay never runs stale values
+    if (b->tuning_gen != ctx->tuning_gen) {
+        const int rc = capture(ctx, b);
+        if (rc) return rc;
+    }
     if (b->exec) {
         RT_HIP(ctx, hipGraphLaunch(b->exec, s));
         return RT_OK;

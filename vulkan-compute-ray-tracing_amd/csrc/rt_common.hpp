@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -152,6 +153,29 @@ static_assert(sizeof(SphereNode) == 32, "SphereNode is 32 bytes");
 constexpr uint32_t kSphereLinear = 16;     // up to this many: a plain loop
 constexpr uint32_t kSphereStack = 32;      // LDS stack entries per lane
 constexpr uint32_t kMaxTreeSpheres = 1u << 24;   // median split: depth <= 24
+
+// Host half of an upload: the scene validated and packed, its device bytes
+// known before anything is allocated on the device (so a caller with an
+// HBM budget can make room first).  scene_commit allocates and copies.
+struct PreparedScene {
+    rt_scene_desc desc{};               // the caller's arrays (the tree the shader order walks)
+    std::vector<rt_bvh_node> built;     // the device-built tree (RT_TREE_DEVICE), or empty
+    SceneCheck chk;
+    PackedTree packed;
+    bool fast = false;
+    bool tree_built = false;
+    bool sphere_tree = false;
+    std::vector<SphereNode> snodes;
+    bool with_pnodes = false, with_qnodes = false;   // binary records uploaded now
+    uint64_t bytes = 0;                 // device bytes scene_commit allocates
+};
+int scene_prepare(rt_ctx* ctx, const rt_scene_desc* desc, PreparedScene& out);
+int scene_commit(rt_ctx* ctx, PreparedScene& prep, rt_scene** out);
+// Device bytes of the 4-wide records and packed triangles as uploaded.
+uint64_t packed_record_bytes(const PackedTree& p);
+// The binary node records of `layout` (RT_LAYOUT_F32 / RT_LAYOUT_Q16) for a
+// scene uploaded without them: rebuilt from the scene's device arrays.
+int scene_binary_records(rt_ctx* ctx, rt_scene* s, uint32_t layout);
 }  // namespace rtamd
 
 struct rt_scene {
@@ -179,6 +203,12 @@ struct rt_scene {
     uint32_t ref_height = 0;
     bool fast = false;
     rtamd::SphereNode* d_snodes = nullptr;   // sphere tree, or null: plain loop
+    // the packed arrays come from a device-built tree (RT_TREE_DEVICE), not
+    // from d_nodes: a later build of the binary records rebuilds that tree
+    bool tree_built = false;
+    // d_pnodes / d_qnodes are built on the first packed query that needs
+    // them (binary layouts); this lock orders those builds
+    std::mutex lazy_mu;
 };
 
 struct rt_ctx {
@@ -196,6 +226,7 @@ struct rt_ctx {
     uint32_t* d_cursors = nullptr;    // ring of persistent-kernel ray cursors
     uint32_t cursor_next = 0;
     rt_tuning tuning{};               // 0 fields = defaults
+    uint64_t tuning_gen = 0;          // bumped by every rt_tuning_set
     uint64_t last_primary = 0, last_secondary = 0;
     // opt-in stage timing of renders: events around generate, each bounce's
     // trace and shade, and store (created on the first timed render)

@@ -455,45 +455,80 @@ int device_tree(rt_ctx* ctx, const rt_triangle* tris, uint32_t n, std::vector<rt
     return rc;
 }
 
-extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
+namespace rtamd {
+
+uint64_t packed_record_bytes(const PackedTree& p) {
+    const uint64_t nb = (sizeof(PNode4) * (uint64_t)p.w4nodes.size() + 255) / 256 * 256;
+    const uint64_t tb = sizeof(PTri) * (uint64_t)p.tris.size();
+    if (nb + tb >= (1ull << 32) || !nb || !tb) return sizeof(PNode4) * (uint64_t)p.w4nodes.size() + tb;
+    return nb + tb;
+}
+
+int scene_prepare(rt_ctx* ctx, const rt_scene_desc* desc, PreparedScene& P) {
     if (!ctx) return RT_ERR_INVALID;
-    if (!desc || !out) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: null argument");
-    *out = nullptr;
-    RT_HIP(ctx, hipSetDevice(ctx->device));
+    if (!desc) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: null argument");
     // `user` keeps the caller's tree (the shader-order traversal walks it),
     // `packsrc` is the tree the packed traversal is built from
     rt_scene_desc user = *desc, packsrc = *desc;
-    std::vector<rt_bvh_node> built;
-    const bool rebuild = ctx->tuning.tree == RT_TREE_DEVICE;
-    if (rebuild) {
-        int rc = device_tree(ctx, desc->triangles, desc->num_triangles, built);
+    P.tree_built = ctx->tuning.tree == RT_TREE_DEVICE;
+    if (P.tree_built) {
+        RT_HIP(ctx, hipSetDevice(ctx->device));
+        int rc = device_tree(ctx, desc->triangles, desc->num_triangles, P.built);
         if (rc) return rc;
-        packsrc.nodes = built.data();
-        packsrc.num_nodes = (uint32_t)built.size();
+        packsrc.nodes = P.built.data();
+        packsrc.num_nodes = (uint32_t)P.built.size();
         if (!desc->nodes || desc->num_nodes == 0) user = packsrc;
     }
-    SceneCheck chk, pchk;
+    SceneCheck pchk;
     try {
-        chk = check_scene(user);
-        pchk = rebuild && !chk.error.empty() ? chk : (rebuild ? check_scene(packsrc) : chk);
+        P.chk = check_scene(user);
+        pchk = P.tree_built && !P.chk.error.empty() ? P.chk : (P.tree_built ? check_scene(packsrc) : P.chk);
     } catch (const std::exception& e) {
         return set_error(ctx, RT_ERR_NOMEM, std::string("rt_scene_upload: ") + e.what());
     }
-    if (!chk.error.empty()) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: " + chk.error);
+    if (!P.chk.error.empty()) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: " + P.chk.error);
     if (!pchk.error.empty()) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: device tree: " + pchk.error);
-
-    PackedTree packed;
-    bool fast = pchk.standard && pchk.enclosing;
-    if (fast) {
+    P.fast = pchk.standard && pchk.enclosing;
+    if (P.fast) {
         try {
-            pack_tree(packsrc, packed);
+            pack_tree(packsrc, P.packed);
         } catch (const std::exception& e) {
             return set_error(ctx, RT_ERR_NOMEM, std::string("rt_scene_upload: ") + e.what());
         }
-        if (packed.height > kMaxPackedHeight) fast = false;
+        if (P.packed.height > kMaxPackedHeight) P.fast = false;
     }
-    desc = &user;
+    P.desc = user;
+    if (user.num_spheres > kSphereLinear && user.num_spheres <= kMaxTreeSpheres) {
+        try {
+            P.sphere_tree = build_sphere_tree(user.spheres, user.num_spheres, P.snodes);
+        } catch (const std::bad_alloc&) {
+            return set_error(ctx, RT_ERR_NOMEM, "rt_scene_upload: sphere tree");
+        }
+    }
+    // binary node records only when this context's tuning reads them; a
+    // later query in such a layout builds them then (scene_binary_records)
+    P.with_pnodes = P.fast && ctx->tuning.node_layout == RT_LAYOUT_F32;
+    P.with_qnodes = P.fast && ctx->tuning.node_layout == RT_LAYOUT_Q16;
+    uint64_t b = sizeof(rt_triangle) * (uint64_t)user.num_triangles + sizeof(rt_material) * (uint64_t)user.num_materials +
+                 sizeof(rt_bvh_node) * (uint64_t)user.num_nodes + sizeof(rt_light) * (uint64_t)user.num_lights +
+                 sizeof(rt_sphere) * (uint64_t)user.num_spheres;
+    if (P.sphere_tree) b += sizeof(SphereNode) * (uint64_t)P.snodes.size();
+    if (P.fast) {
+        b += packed_record_bytes(P.packed) + sizeof(int32_t) * (uint64_t)P.packed.tri_id.size();
+        if (P.with_pnodes) b += sizeof(PNode) * (uint64_t)P.packed.nodes.size();
+        if (P.with_qnodes) b += sizeof(PNodeQ) * (uint64_t)P.packed.qnodes.size();
+    }
+    P.bytes = b;
+    return RT_OK;
+}
 
+int scene_commit(rt_ctx* ctx, PreparedScene& P, rt_scene** out) {
+    if (!ctx || !out) return RT_ERR_INVALID;
+    *out = nullptr;
+    RT_HIP(ctx, hipSetDevice(ctx->device));
+    const rt_scene_desc* desc = &P.desc;
+    const PackedTree& packed = P.packed;
+    const bool fast = P.fast;
     rt_scene* s = new rt_scene();
     s->ctx = ctx;
     uint64_t bytes = 0;
@@ -504,20 +539,13 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene*
         if ((rc = upload_array(ctx, &s->d_nodes, desc->nodes, desc->num_nodes, bytes))) break;
         if ((rc = upload_array(ctx, &s->d_lights, desc->lights, desc->num_lights, bytes))) break;
         if ((rc = upload_array(ctx, &s->d_spheres, desc->spheres, desc->num_spheres, bytes))) break;
-        if (desc->num_spheres > kSphereLinear && desc->num_spheres <= kMaxTreeSpheres) {
-            std::vector<SphereNode> snodes;
-            bool built = false;
-            try {
-                built = build_sphere_tree(desc->spheres, desc->num_spheres, snodes);
-            } catch (const std::bad_alloc&) {
-                rc = set_error(ctx, RT_ERR_NOMEM, "rt_scene_upload: sphere tree");
-                break;
-            }
-            if (built && (rc = upload_array(ctx, &s->d_snodes, snodes.data(), (uint32_t)snodes.size(), bytes))) break;
-        }
+        if (P.sphere_tree &&
+            (rc = upload_array(ctx, &s->d_snodes, P.snodes.data(), (uint32_t)P.snodes.size(), bytes))) break;
         if (fast) {
-            if ((rc = upload_array(ctx, &s->d_pnodes, packed.nodes.data(), (uint32_t)packed.nodes.size(), bytes))) break;
-            if ((rc = upload_array(ctx, &s->d_qnodes, packed.qnodes.data(), (uint32_t)packed.qnodes.size(), bytes))) break;
+            if (P.with_pnodes &&
+                (rc = upload_array(ctx, &s->d_pnodes, packed.nodes.data(), (uint32_t)packed.nodes.size(), bytes))) break;
+            if (P.with_qnodes &&
+                (rc = upload_array(ctx, &s->d_qnodes, packed.qnodes.data(), (uint32_t)packed.qnodes.size(), bytes))) break;
             if ((rc = upload_records(ctx, s, packed, bytes))) break;
             if ((rc = upload_array(ctx, &s->d_ptri_id, packed.tri_id.data(), (uint32_t)packed.tri_id.size(), bytes))) break;
         }
@@ -531,21 +559,78 @@ extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene*
     s->root4 = fast ? packed.root4 : kEmptyRoot;
     s->height4 = fast ? packed.height4 : 0;
     s->packed_height = fast ? packed.height : 0;
-    s->ref_height = chk.height;
+    s->ref_height = P.chk.height;
     s->fast = fast;
+    s->tree_built = P.tree_built;
     rt_scene_info& in = s->info;
     in.num_triangles = desc->num_triangles;
     in.num_nodes = desc->num_nodes;
     in.num_lights = desc->num_lights;
     in.num_spheres = desc->num_spheres;
     in.num_materials = desc->num_materials;
-    in.tree_height = chk.height;
-    in.reachable_leaves = chk.leaves;
-    in.enclosing = chk.enclosing ? 1u : 0u;
+    in.tree_height = P.chk.height;
+    in.reachable_leaves = P.chk.leaves;
+    in.enclosing = P.chk.enclosing ? 1u : 0u;
     in.fast_path = fast ? 1u : 0u;
     in.device_bytes = bytes;
     *out = s;
     return RT_OK;
+}
+
+int scene_binary_records(rt_ctx* ctx, rt_scene* s, uint32_t layout) {
+    if (!s->fast) return RT_OK;
+    std::lock_guard<std::mutex> lk(s->lazy_mu);
+    const bool want_f = layout == RT_LAYOUT_F32 && !s->d_pnodes;
+    const bool want_q = layout == RT_LAYOUT_Q16 && !s->d_qnodes;
+    if (!want_f && !want_q) return RT_OK;
+    RT_HIP(ctx, hipSetDevice(ctx->device));
+    // the tree the packed arrays came from, read back: the scene's own node
+    // array, or the device builder's tree over its triangles
+    const uint32_t nt = s->info.num_triangles;
+    std::vector<rt_triangle> tris;
+    std::vector<rt_bvh_node> nodes;
+    try {
+        tris.resize(nt);
+        if (!s->tree_built) nodes.resize(s->info.num_nodes);
+    } catch (const std::bad_alloc&) {
+        return set_error(ctx, RT_ERR_NOMEM, "binary node records: host copy");
+    }
+    RT_HIP(ctx, hipMemcpy(tris.data(), s->d_tris, sizeof(rt_triangle) * (size_t)nt, hipMemcpyDeviceToHost));
+    if (s->tree_built) {
+        int rc = device_tree(ctx, tris.data(), nt, nodes);
+        if (rc) return rc;
+    } else {
+        RT_HIP(ctx, hipMemcpy(nodes.data(), s->d_nodes, sizeof(rt_bvh_node) * nodes.size(), hipMemcpyDeviceToHost));
+    }
+    rt_scene_desc d{};
+    d.triangles = tris.data();
+    d.num_triangles = nt;
+    d.nodes = nodes.data();
+    d.num_nodes = (uint32_t)nodes.size();
+    PackedTree packed;
+    try {
+        pack_tree(d, packed);
+    } catch (const std::exception& e) {
+        return set_error(ctx, RT_ERR_NOMEM, std::string("binary node records: ") + e.what());
+    }
+    uint64_t bytes = 0;
+    int rc = want_f ? upload_array(ctx, &s->d_pnodes, packed.nodes.data(), (uint32_t)packed.nodes.size(), bytes)
+                    : upload_array(ctx, &s->d_qnodes, packed.qnodes.data(), (uint32_t)packed.qnodes.size(), bytes);
+    if (rc) return rc;
+    s->info.device_bytes += bytes;
+    return RT_OK;
+}
+
+}  // namespace rtamd
+
+extern "C" int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
+    if (!ctx) return RT_ERR_INVALID;
+    if (!desc || !out) return set_error(ctx, RT_ERR_INVALID, "rt_scene_upload: null argument");
+    *out = nullptr;
+    PreparedScene prep;
+    int rc = scene_prepare(ctx, desc, prep);
+    if (rc) return rc;
+    return scene_commit(ctx, prep, out);
 }
 
 extern "C" int rt_scene_check(const rt_scene_desc* desc, rt_scene_info* info, char* err, size_t errlen) {

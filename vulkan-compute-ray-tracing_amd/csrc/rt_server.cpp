@@ -52,6 +52,7 @@ struct Request {
     int status = RT_OK;
     bool done = false;
     bool detached = false;       // nobody waits: freed on completion
+    bool running = false;        // taken by a slot
     bool direct = false;         // caller buffers are page-locked: DMA in place
     uint32_t offset = 0;         // first ray in the slot's staging
 };
@@ -107,7 +108,8 @@ struct rt_server {
     std::unordered_map<uint64_t, Request*> waited;
     std::unordered_map<uint64_t, SrvScene> scenes;
     uint64_t next_request = 1, next_scene = 1, tick = 1;
-    uint64_t resident = 0;
+    uint64_t resident = 0;               // admitted scene bytes (includes one being uploaded)
+    uint64_t peak = 0;                   // most of them at once
     uint64_t in_flight = 0;              // submitted and not completed
     bool stop = false;
     rt_server_stats st{};
@@ -250,6 +252,8 @@ void finish_request(rt_server* S, Request* r, int status) {
     if (status == RT_OK) {
         S->st.completed++;
         S->lat_ms.push_back((float)std::chrono::duration<double, std::milli>(r->t1 - r->t0).count());
+    } else if (status == RT_ERR_CANCELLED) {
+        S->st.cancelled++;
     } else {
         S->st.failed++;
     }
@@ -297,6 +301,20 @@ void worker_main(rt_server* S) {
             }
         }
         if (S->stop && S->pending.empty() && !busy) break;
+        // requests queued past their deadline are dropped unrun; every
+        // request has the same deadline, so they are the oldest, in front
+        if (S->cfg.deadline_ms && !S->pending.empty()) {
+            const auto now = Clock::now();
+            const auto limit = std::chrono::milliseconds(S->cfg.deadline_ms);
+            bool dropped = false;
+            while (!S->pending.empty() && now - S->pending.front()->t0 >= limit) {
+                Request* r = S->pending.front();
+                S->pending.pop_front();
+                finish_request(S, r, RT_ERR_CANCELLED);
+                dropped = true;
+            }
+            if (dropped) S->done_cv.notify_all();
+        }
         for (Slot& sl : S->slots) {
             if (sl.busy || S->pending.empty()) continue;
             sl.render = S->pending.front()->kind == 1;
@@ -313,6 +331,7 @@ void worker_main(rt_server* S) {
                     S->pending.pop_front();
                 }
             }
+            for (Request* r : sl.reqs) r->running = true;
             sl.busy = true;
             S->st.launches++;
             lk.unlock();
@@ -419,50 +438,99 @@ extern "C" int rt_server_destroy(rt_server* S) {
 
 extern "C" int rt_server_scene_add(rt_server* S, const rt_scene_desc* desc, uint64_t* scene_id) {
     if (!S || !desc || !scene_id) return RT_ERR_INVALID;
-    rt_scene* sc = nullptr;
+    // the host half first: validated and packed, so the scene's device bytes
+    // are known before anything is allocated on the device
+    PreparedScene prep;
     {
-        // uploads run outside the request lock so submissions never wait on them
         std::lock_guard<std::mutex> up(S->upload_mu);
-        int rc = rt_scene_upload(S->ctx, desc, &sc);
+        int rc = scene_prepare(S->ctx, desc, prep);
         if (rc) return fail(S, rc, std::string("rt_server_scene_add: ") + rt_last_error(S->ctx));
-        if (!sc->fast) {
-            rt_scene_release(S->ctx, sc);
-            return fail(S, RT_ERR_UNSUPPORTED, "rt_server_scene_add: scene without the packed traversal");
-        }
+        if (!prep.fast) return fail(S, RT_ERR_UNSUPPORTED, "rt_server_scene_add: scene without the packed traversal");
     }
+    const uint64_t bytes = prep.bytes;
+    const uint64_t budget = S->cfg.hbm_budget;
+    // the least recently used scene without queued or running work
+    auto pick_victim = [&]() -> uint64_t {
+        uint64_t victim = 0, oldest = UINT64_MAX;
+        for (auto& kv : S->scenes)
+            if (kv.second.refs == 0 && kv.second.last_use < oldest) { oldest = kv.second.last_use; victim = kv.first; }
+        return victim;
+    };
+    auto evict = [&](uint64_t victim, std::vector<rt_scene*>& out) {
+        out.push_back(S->scenes[victim].scene);
+        S->resident -= S->scenes[victim].bytes;
+        S->scenes.erase(victim);
+        S->st.scenes_evicted++;
+    };
+    // admission under the budget: evict, then reserve the new scene's bytes
     std::vector<rt_scene*> evicted;
-    int rc = RT_OK;
+    bool admitted = false;
     {
         std::lock_guard<std::mutex> lk(S->mu);
-        const uint64_t bytes = sc->info.device_bytes;
-        const uint64_t budget = S->cfg.hbm_budget;
         while (budget && S->resident + bytes > budget) {
-            uint64_t victim = 0, oldest = UINT64_MAX;
-            for (auto& kv : S->scenes)
-                if (kv.second.refs == 0 && kv.second.last_use < oldest) { oldest = kv.second.last_use; victim = kv.first; }
-            if (!victim) break;
-            evicted.push_back(S->scenes[victim].scene);
-            S->resident -= S->scenes[victim].bytes;
-            S->scenes.erase(victim);
-            S->st.scenes_evicted++;
+            const uint64_t v = pick_victim();
+            if (!v) break;
+            evict(v, evicted);
         }
-        if (budget && S->resident + bytes > budget) {
-            S->st.scenes_refused++;
-            rc = fail(S, RT_ERR_NOMEM, "rt_server_scene_add: HBM budget exceeded by scenes with pending work");
-        } else {
-            uint64_t id = S->next_scene++;
-            S->scenes[id] = SrvScene{sc, bytes, S->tick++, 0};
+        admitted = !budget || S->resident + bytes <= budget;
+        if (admitted) {
             S->resident += bytes;
-            S->st.scenes_added++;
-            *scene_id = id;
+            S->peak = std::max(S->peak, S->resident);
+        } else {
+            S->st.scenes_refused++;
         }
     }
-    if (rc) evicted.push_back(sc);
-    {
-        std::lock_guard<std::mutex> up(S->upload_mu);
+    std::lock_guard<std::mutex> up(S->upload_mu);
+    for (rt_scene* e : evicted) rt_scene_release(S->ctx, e);      // freed before the new scene is allocated
+    evicted.clear();
+    if (!admitted) return fail(S, RT_ERR_NOMEM, "rt_server_scene_add: HBM budget exceeded by scenes with pending work");
+    // the device itself must hold the scene too (other users share it):
+    // make room by evicting further idle scenes while its free memory is short
+    const uint64_t margin = 64ull << 20;
+    for (;;) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || (uint64_t)free_b >= bytes + margin) break;
+        uint64_t v = 0;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            v = pick_victim();
+            if (v) evict(v, evicted);
+        }
+        if (!v) break;
         for (rt_scene* e : evicted) rt_scene_release(S->ctx, e);
+        evicted.clear();
     }
-    return rc;
+    rt_scene* sc = nullptr;
+    int rc = scene_commit(S->ctx, prep, &sc);
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (rc) {
+        S->resident -= bytes;
+        S->st.scenes_refused++;
+        return fail(S, rc == RT_ERR_HIP ? RT_ERR_NOMEM : rc,
+                    std::string("rt_server_scene_add: upload: ") + rt_last_error(S->ctx));
+    }
+    uint64_t id = S->next_scene++;
+    S->scenes[id] = SrvScene{sc, bytes, S->tick++, 0};
+    S->st.scenes_added++;
+    *scene_id = id;
+    return RT_OK;
+}
+
+extern "C" int rt_server_cancel(rt_server* S, uint64_t id) {
+    if (!S) return RT_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (auto it = S->pending.begin(); it != S->pending.end(); ++it) {
+        if ((*it)->id != id) continue;
+        Request* r = *it;
+        S->pending.erase(it);
+        finish_request(S, r, RT_ERR_CANCELLED);
+        S->done_cv.notify_all();
+        return RT_OK;
+    }
+    auto w = S->waited.find(id);
+    if (w != S->waited.end() && w->second->running && !w->second->done)
+        return fail(S, RT_ERR_BUSY, "rt_server_cancel: request is already on the device");
+    return fail(S, RT_ERR_NOT_FOUND, "rt_server_cancel: no queued request with this id");
 }
 
 extern "C" int rt_server_scene_remove(rt_server* S, uint64_t id) {
@@ -554,6 +622,7 @@ extern "C" int rt_server_stats_get(rt_server* S, rt_server_stats* out) {
     std::lock_guard<std::mutex> lk(S->mu);
     *out = S->st;
     out->resident_bytes = S->resident;
+    out->peak_resident_bytes = S->peak;
     if (!S->lat_ms.empty()) {
         std::vector<float> v = S->lat_ms;
         std::sort(v.begin(), v.end());

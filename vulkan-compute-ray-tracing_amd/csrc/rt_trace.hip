@@ -308,6 +308,13 @@ int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s) {
         if (need <= 96) return launch_w4<96>(ctx, a, s, stats);
         return set_error(ctx, RT_ERR_UNSUPPORTED, "4-wide tree deeper than 31 levels");
     }
+    // binary records are built on first use (an upload under the 4-wide
+    // layout leaves them out); the scene's arrays are immutable otherwise
+    const uint32_t layout = (uint32_t)resolved_layout(ctx);
+    if ((layout == RT_LAYOUT_F32 && !sc->d_pnodes) || (layout == RT_LAYOUT_Q16 && !sc->d_qnodes)) {
+        const int rc = scene_binary_records(ctx, const_cast<rt_scene*>(sc), layout);
+        if (rc) return rc;
+    }
     const uint32_t h = sc->packed_height;
     if (h <= 16) return launch_packed_layout<16>(ctx, a, s, stats);
     if (h <= 20) return launch_packed_layout<20>(ctx, a, s, stats);

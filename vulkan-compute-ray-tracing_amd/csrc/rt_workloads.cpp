@@ -372,10 +372,24 @@ extern "C" int rtw_rays_box(uint32_t n, uint64_t seed, float extent, rt_ray* out
 // ---------------------------------------------------------------------------
 // Poisson request stream against a server (rtamd_workloads.h).
 extern "C" int rtw_serve_poisson(rt_server* S, const rtw_serve_params* p, rtw_serve_result* res) {
+    return rtw_serve_poisson_sampled(S, p, nullptr, res);
+}
+
+extern "C" int rtw_serve_poisson_sampled(rt_server* S, const rtw_serve_params* p, rtw_serve_sampling* smp,
+                                         rtw_serve_result* res) {
     using Clock = std::chrono::steady_clock;
     if (!S || !p || !res || p->rate_per_s <= 0.0 || p->duration_s <= 0.0 || p->query_min == 0 ||
         p->query_max < p->query_min || p->render_w == 0 || p->render_h == 0)
         return RT_ERR_INVALID;
+    if (smp) {
+        if (smp->every == 0 || (smp->capacity && !smp->samples) || (smp->swaps_capacity && !smp->swap_ids))
+            return RT_ERR_INVALID;
+        for (uint32_t i = 0; i < smp->capacity; ++i)
+            if (!smp->samples[i].hits || !smp->samples[i].image) return RT_ERR_INVALID;
+        smp->taken = 0;
+        smp->swaps_taken = 0;
+        for (uint32_t i = 0; i < smp->swaps_capacity; ++i) smp->swap_ids[i] = 0;   // 0: that swap failed
+    }
     std::memset(res, 0, sizeof *res);
     rt_server_stats_reset(S);
     SplitMix rng(p->seed);
@@ -422,7 +436,13 @@ extern "C" int rtw_serve_poisson(rt_server* S, const rtw_serve_params* p, rtw_se
                 rt_scene_desc d{ws.triangles, ws.num_triangles, ws.materials, ws.num_materials, ws.nodes, ws.num_nodes,
                                 ws.lights, ws.num_lights, ws.spheres, ws.num_spheres};
                 uint64_t id = 0;
-                if (rt_server_scene_add(S, &d, &id) == RT_OK) swaps++;
+                if (rt_server_scene_add(S, &d, &id) == RT_OK) {
+                    swaps++;
+                    if (smp && k < smp->swaps_capacity) {
+                        smp->swap_ids[k] = id;
+                        smp->swaps_taken = k + 1;
+                    }
+                }
                 rtw_scene_free(&ws);
                 refresh();
             }
@@ -442,6 +462,11 @@ extern "C" int rtw_serve_poisson(rt_server* S, const rtw_serve_params* p, rtw_se
         const bool render = (double)(rng.next() >> 11) * 0x1p-53 < p->render_fraction;
         const double us = (double)(rng.next() >> 11) * 0x1p-53;
         const double un = (double)(rng.next() >> 11) * 0x1p-53;
+        // a kept arrival writes to its own buffers, which no later request reuses
+        rtw_serve_sample* keep =
+            (smp && k % smp->every == 0 && smp->taken < smp->capacity) ? &smp->samples[smp->taken] : nullptr;
+        rt_hit* hits_out = keep ? keep->hits : hit_ring[k % kRing];
+        uint8_t* image_out = keep ? keep->image : tgt_ring[k % kRing];
         for (int attempt = 0; attempt < 2; ++attempt) {
             uint64_t id;
             {
@@ -450,20 +475,31 @@ extern "C" int rtw_serve_poisson(rt_server* S, const rtw_serve_params* p, rtw_se
                 id = ids[std::min(ids.size() - 1, (size_t)(us * (double)ids.size()))];
             }
             int rc;
+            uint32_t n = 0;
             if (render) {
                 rt_ubo ubo{};
                 ubo.camPos[1] = 10.0f;            // eye at (0, 0, 10)
                 rt_render_params rp{};
                 rp.vfov_deg = 60.0f;
                 rp.bounces = p->render_bounces;
-                rc = rt_server_submit_render(S, id, &ubo, &rp, p->render_w, p->render_h, tgt_ring[k % kRing], nullptr);
+                rc = rt_server_submit_render(S, id, &ubo, &rp, p->render_w, p->render_h, image_out, nullptr);
             } else {
-                uint32_t n = (uint32_t)std::floor(std::exp(un * lq) * (double)p->query_min);
+                n = (uint32_t)std::floor(std::exp(un * lq) * (double)p->query_min);
                 n = std::max(p->query_min, std::min(p->query_max, n));
-                rc = rt_server_submit_query(S, id, pool, n, hit_ring[k % kRing], nullptr);
+                rc = rt_server_submit_query(S, id, pool, n, hits_out, nullptr);
             }
             if (rc == RT_ERR_NOT_FOUND) { refresh(); continue; }   // the scene was just evicted
-            if (rc == RT_OK) { res->requests++; (render ? res->renders : res->queries)++; }
+            if (rc == RT_OK) {
+                res->requests++;
+                (render ? res->renders : res->queries)++;
+                if (keep) {
+                    keep->scene_id = id;
+                    keep->arrival = k;
+                    keep->kind = render ? 1u : 0u;
+                    keep->count = n;
+                    smp->taken++;
+                }
+            }
             break;
         }
         ++k;

@@ -123,13 +123,14 @@ class QueryStats(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("slots", ctypes.c_uint32), ("max_batch_rays", ctypes.c_uint32),
                 ("max_render_pixels", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("deadline_ms", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "failed", "launches", "scenes_added",
                                                "scenes_evicted", "scenes_refused", "rays", "resident_bytes")] + \
-               [(k, ctypes.c_double) for k in ("p50_ms", "p99_ms", "mean_ms", "max_ms")]
+               [(k, ctypes.c_double) for k in ("p50_ms", "p99_ms", "mean_ms", "max_ms")] + \
+               [(k, ctypes.c_uint64) for k in ("peak_resident_bytes", "cancelled")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -146,6 +147,18 @@ class ServeParams(ctypes.Structure):
 class ServeResult(ctypes.Structure):
     _fields_ = [("stats", ServerStats), ("wall_s", ctypes.c_double), ("requests", ctypes.c_uint64),
                 ("queries", ctypes.c_uint64), ("renders", ctypes.c_uint64), ("swaps", ctypes.c_uint64)]
+
+
+class ServeSample(ctypes.Structure):
+    _fields_ = [("hits", ctypes.c_void_p), ("image", ctypes.c_void_p), ("scene_id", ctypes.c_uint64),
+                ("arrival", ctypes.c_uint64), ("kind", ctypes.c_uint32), ("count", ctypes.c_uint32)]
+
+
+class ServeSampling(ctypes.Structure):
+    _fields_ = [("every", ctypes.c_uint32), ("capacity", ctypes.c_uint32), ("samples", ctypes.POINTER(ServeSample)),
+                ("taken", ctypes.c_uint32), ("swaps_capacity", ctypes.c_uint32),
+                ("swap_ids", ctypes.POINTER(ctypes.c_uint64)), ("swaps_taken", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 SERVE_BATCHED, SERVE_FIFO = 0, 1
@@ -178,8 +191,8 @@ EXPORTS = [
     "rt_batch_create", "rt_batch_set_config", "rt_batch_get_config", "rt_batch_launch", "rt_batch_autotune",
     "rt_batch_destroy",
     "rt_host_alloc", "rt_host_free", "rt_server_create", "rt_server_destroy", "rt_server_scene_add", "rt_server_scene_remove", "rt_server_scene_list",
-    "rt_server_submit_query", "rt_server_submit_render", "rt_server_wait", "rt_server_drain", "rt_server_stats_get",
-    "rt_server_stats_reset", "rt_server_last_error", "rtw_serve_poisson",
+    "rt_server_submit_query", "rt_server_submit_render", "rt_server_wait", "rt_server_cancel", "rt_server_drain", "rt_server_stats_get",
+    "rt_server_stats_reset", "rt_server_last_error", "rtw_serve_poisson", "rtw_serve_poisson_sampled",
     "rtw_scene_soup", "rtw_scene_icosphere", "rtw_scene_clusters", "rtw_scene_doge", "rtw_scene_free",
     "rt_obj_load", "rt_obj_last_error", "rt_free", "rtw_rays_camera_tiles",
     "rtw_rays_shell", "rtw_rays_box", "rtw_log_uniform",
@@ -238,11 +251,14 @@ def lib():
         "rt_server_submit_query": (i32, [P, u64, P, u32, P, P]),
         "rt_server_submit_render": (i32, [P, u64, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), u32, u32, P, P]),
         "rt_server_wait": (i32, [P, u64, i32, ctypes.POINTER(i32)]),
+        "rt_server_cancel": (i32, [P, u64]),
         "rt_server_drain": (i32, [P]),
         "rt_server_stats_get": (i32, [P, ctypes.POINTER(ServerStats)]),
         "rt_server_stats_reset": (i32, [P]),
         "rt_server_last_error": (ctypes.c_char_p, [P]),
         "rtw_serve_poisson": (i32, [P, ctypes.POINTER(ServeParams), ctypes.POINTER(ServeResult)]),
+        "rtw_serve_poisson_sampled": (i32, [P, ctypes.POINTER(ServeParams), ctypes.POINTER(ServeSampling),
+                                            ctypes.POINTER(ServeResult)]),
         "rt_bvh_build": (i32, [P, u32, P]),
         "rt_bvh_build_seeded": (i32, [P, u32, u32, u32, P]),
         "rt_bvh_build_device": (i32, [P, P, u32, P, P]),
@@ -704,9 +720,9 @@ class Server:
     """An rt_server: resident scenes and a request loop on its own thread."""
 
     def __init__(self, ctx: "Context", mode=SERVE_BATCHED, slots=0, max_batch_rays=0, max_render_pixels=0,
-                 hbm_budget=0):
+                 hbm_budget=0, deadline_ms=0):
         self.ctx = ctx
-        cfg = ServerConfig(mode, slots, max_batch_rays, max_render_pixels, hbm_budget)
+        cfg = ServerConfig(mode, slots, max_batch_rays, max_render_pixels, hbm_budget, deadline_ms)
         h = ctypes.c_void_p()
         ctx.check(lib().rt_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)), "rt_server_create")
         self.handle = h
@@ -754,6 +770,11 @@ class Server:
         self._keep.pop(rid, None)
         return st.value
 
+    def cancel(self, rid: int) -> int:
+        """Cancels a queued request; returns the status (RT_OK, RT_ERR_BUSY
+        when it is already running, RT_ERR_NOT_FOUND when unknown or done)."""
+        return lib().rt_server_cancel(self.handle, rid)
+
     def drain(self):
         self._check(lib().rt_server_drain(self.handle), "rt_server_drain")
 
@@ -772,6 +793,40 @@ class Server:
         r = ServeResult()
         self._check(lib().rtw_serve_poisson(self.handle, ctypes.byref(p), ctypes.byref(r)), "rtw_serve_poisson")
         return r
+
+    def poisson_sampled(self, rate, duration, every, capacity, swaps_capacity=16, render_fraction=0.1,
+                        swap_interval=10.0, query_min=1000, query_max=262144, render_w=512, render_h=512,
+                        render_bounces=4, swap_triangles=500000, seed=7, threads=16):
+        """A Poisson run that keeps every `every`-th arrival's result (up to
+        `capacity`).  Returns (result, samples, swap_ids): each sample is a
+        dict with kind ("query"/"render"), scene_id, arrival, count and its
+        hits (HIT_DT records) or RGBA8 image; swap_ids[k] is the server id
+        of rtw_scene_clusters(swap_triangles, seed + 1000 + k) (0 if that
+        swap failed)."""
+        p = ServeParams(rate, duration, render_fraction, swap_interval, query_min, query_max, render_w, render_h,
+                        render_bounces, swap_triangles, seed, threads, 0)
+        hits = [np.zeros(query_max, HIT_DT) for _ in range(capacity)]
+        imgs = [np.zeros((render_h, render_w, 4), np.uint8) for _ in range(capacity)]
+        arr = (ServeSample * max(1, capacity))()
+        for i in range(capacity):
+            arr[i].hits = hits[i].ctypes.data
+            arr[i].image = imgs[i].ctypes.data
+        ids = (ctypes.c_uint64 * max(1, swaps_capacity))()
+        smp = ServeSampling(every, capacity, arr, 0, swaps_capacity, ids, 0, 0)
+        r = ServeResult()
+        self._check(lib().rtw_serve_poisson_sampled(self.handle, ctypes.byref(p), ctypes.byref(smp), ctypes.byref(r)),
+                    "rtw_serve_poisson_sampled")
+        samples = []
+        for i in range(smp.taken):
+            e = arr[i]
+            d = {"kind": "render" if e.kind else "query", "scene_id": int(e.scene_id), "arrival": int(e.arrival),
+                 "count": int(e.count)}
+            if e.kind:
+                d["image"] = imgs[i]
+            else:
+                d["hits"] = hits[i][:e.count]
+            samples.append(d)
+        return r, samples, [int(ids[k]) for k in range(smp.swaps_taken)]
 
     def close(self):
         if self.handle:
