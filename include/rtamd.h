@@ -182,6 +182,20 @@ int rt_closest_hit(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays_dev,
                    uint32_t n, rt_hit* hits_dev, const rt_query_params* params,
                    void* stream);
 
+/* rt_closest_hit without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under stream
+ * capture, serving loops and harnesses that forbid hipMalloc).  Its
+ * scratch (ray cursors, the trace-order sort) is one static device array
+ * per device of about 81 MiB, held until the process ends.  Limits: n <=
+ * 4,194,304 (RT_ERR_INVALID above); the packed traversal only, whole stack
+ * in LDS (stack_lds 0), binary layouts only once their records exist
+ * (RT_ERR_UNSUPPORTED otherwise).  Calls on one device share the scratch:
+ * calls that overlap on one device race, so order them on one stream.
+ * Hits equal rt_closest_hit's byte for byte. */
+int rt_closest_hit_noalloc(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays_dev,
+                           uint32_t n, rt_hit* hits_dev, const rt_query_params* params,
+                           void* stream);
+
 /* Per-ray traversal counters of the packed kernel (node visits = 64-byte
  * node records fetched, tri tests = 48-byte triangle records fetched). */
 int rt_closest_hit_stats(rt_ctx* ctx, const rt_scene* scene, const rt_ray* rays_dev,

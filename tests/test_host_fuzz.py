@@ -14,7 +14,7 @@ PKG = os.path.join(ROOT, "vulkan-compute-ray-tracing_amd")
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 HOST = ["rt_scene", "rt_obj", "rt_bvh", "rt_image", "rt_workloads"]
-DEVICE = ["rt_api", "rt_trace", "rt_render", "rt_batch", "rt_build", "rt_frame", "rt_sphere", "rt_server"]
+DEVICE = ["rt_api", "rt_trace", "rt_render", "rt_batch", "rt_build", "rt_frame", "rt_sphere", "rt_server", "rt_noalloc"]
 SAN = ["-fsanitize=address,undefined,float-cast-overflow", "-fno-omit-frame-pointer"]
 
 pytestmark = pytest.mark.slow

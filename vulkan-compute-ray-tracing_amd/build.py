@@ -40,6 +40,7 @@ SOURCES = [
     "rt_image.cpp",
     "rt_obj.cpp",
     "rt_server.cpp",
+    "rt_noalloc.hip",
 ]
 
 COMMON = [

@@ -279,6 +279,8 @@ struct TraceArgs {
     uint32_t* tri_tests;
     bool render = false;        // a bounce of the render's wavefront loop
     const uint32_t* order = nullptr;   // packed traversal: trace order, or index order
+    uint32_t* cursor = nullptr;        // kCursorBlock words for the ray cursors, or null: ctx's ring
+    bool no_alloc = false;             // a variant that must not allocate: refuse what would
 };
 int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
 // The trace order of a packed query (rt_trace.hip): sorts the rays by
@@ -286,6 +288,15 @@ int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
 // for it (or, by default, for large batches on large scenes); *order stays
 // null for index order.
 int ray_order(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, const uint32_t** order);
+// The same into a caller's workspace of ws_bytes (256-byte aligned); an
+// error when the sort does not fit in it.
+int ray_order_in(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, char* ws, size_t ws_bytes, const uint32_t** order);
+// Workspace bytes ray_order_in needs for n rays (query keys), through
+// *bytes; needs the context's device current.
+int ray_order_bytes(rt_ctx* ctx, uint32_t n, size_t* bytes);
+// Scratch of the allocation-free query (rt_noalloc.hip), set up once per
+// device when a context is created.
+int noalloc_setup(int device);
 // Idle lanes that make a drained wave share its busy lanes' work
 // (rt_tuning.steal, default 32).
 uint32_t drain_steal(const rt_ctx* ctx);

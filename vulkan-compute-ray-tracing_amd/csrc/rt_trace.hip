@@ -198,11 +198,13 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
     }
     int32_t* spill = nullptr;
     if (RING) {
+        if (a.no_alloc)
+            return set_error(ctx, RT_ERR_UNSUPPORTED, "an LDS stack ring spills to a workspace: set stack_lds to 0");
         int rc = ensure_workspace(ctx, kWsSpill, (size_t)(STACK - RING) * blocks * kBlock * sizeof(int32_t));
         if (rc) return rc;
         spill = (int32_t*)ctx->ws[kWsSpill];
     }
-    uint32_t* cursor = next_cursor(ctx);
+    uint32_t* cursor = a.cursor ? a.cursor : next_cursor(ctx);
     int rc = launch_fill_u32(ctx, cursor, 0u, kCursorBlock, s);
     if (rc) return rc;
     // segment the ray range per XCD when there is enough of it
@@ -251,42 +253,82 @@ int launch_w4(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
 // 40 6.31, 48 6.44, never 7.63.
 uint32_t drain_steal(const rt_ctx* ctx) { return ctx->tuning.steal ? ctx->tuning.steal : 32u; }
 
-// The sorted order of a query's rays in ctx's order workspace, or null when
-// the batch keeps index order.
-int ray_order(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, const uint32_t** order) {
-    *order = nullptr;
+namespace {
+
+// Whether a packed query is traced in sorted order, and the sort's key bits.
+bool order_wanted(const rt_ctx* ctx, const TraceArgs& a) {
     const uint32_t mode = ctx->tuning.ray_order;
-    if (mode == RT_ORDER_GIVEN || !a.scene->fast) return RT_OK;
+    if (mode == RT_ORDER_GIVEN || !a.scene->fast) return false;
     // by default only where it pays: many rays, and a tree far larger than
     // the caches (a small scene's nodes stay in L2 in any order).  For a
     // render queue `n` is its capacity.
-    if (mode == RT_ORDER_AUTO && (a.n < kOrderMinRays || a.scene->info.num_triangles < kOrderMinTriangles))
-        return RT_OK;
-    const uint32_t n = a.n;
-    // a render queue's keys carry a top bit for the dead slots past its live
-    // count; 7 bits per axis keep them at 22 bits, three 8-bit radix passes
-    const uint32_t key_bits = a.n_dev ? 3u * kQueueOrderBits + 1u : 3u * kOrderBits;
-    size_t sort_bytes = 0;
-    RT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                          (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0u, key_bits, s));
-    const size_t vec = align256(sizeof(uint32_t) * (size_t)n);
-    int rc = ensure_workspace(ctx, kWsOrder, 4 * vec + align256(sort_bytes));
+    return !(mode == RT_ORDER_AUTO && (a.n < kOrderMinRays || a.scene->info.num_triangles < kOrderMinTriangles));
+}
+
+// a render queue's keys carry a top bit for the dead slots past its live
+// count; 7 bits per axis keep them at 22 bits, three 8-bit radix passes
+uint32_t order_key_bits(bool queue) { return queue ? 3u * kQueueOrderBits + 1u : 3u * kOrderBits; }
+
+// Workspace of a sort of n rays: keys and ray indices, each double
+// buffered, then the radix sort's temporary storage.
+int order_bytes(rt_ctx* ctx, uint32_t n, bool queue, size_t* vec, size_t* sort_bytes) {
+    *vec = align256(sizeof(uint32_t) * (size_t)n);
+    rocprim::double_buffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
+    *sort_bytes = 0;
+    RT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, *sort_bytes, k, v, n, 0u, order_key_bits(queue), (hipStream_t)0));
+    return RT_OK;
+}
+
+}  // namespace
+
+int ray_order_bytes(rt_ctx* ctx, uint32_t n, size_t* bytes) {
+    size_t vec = 0, sort_bytes = 0;
+    const int rc = order_bytes(ctx, n, false, &vec, &sort_bytes);
     if (rc) return rc;
-    char* ws = (char*)ctx->ws[kWsOrder];
+    *bytes = 4 * vec + align256(sort_bytes);
+    return RT_OK;
+}
+
+int ray_order_in(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, char* ws, size_t ws_bytes, const uint32_t** order) {
+    *order = nullptr;
+    if (!order_wanted(ctx, a)) return RT_OK;
+    const uint32_t n = a.n;
+    const bool queue = a.n_dev != nullptr;
+    size_t vec = 0, sort_bytes = 0;
+    int rc = order_bytes(ctx, n, queue, &vec, &sort_bytes);
+    if (rc) return rc;
+    if (!ws) {
+        rc = ensure_workspace(ctx, kWsOrder, 4 * vec + align256(sort_bytes));
+        if (rc) return rc;
+        ws = (char*)ctx->ws[kWsOrder];
+    } else if (4 * vec + align256(sort_bytes) > ws_bytes) {
+        return set_error(ctx, RT_ERR_UNSUPPORTED, "ray order: the sort of " + std::to_string(n) + " rays needs " +
+                                                      std::to_string(4 * vec + align256(sort_bytes)) +
+                                                      " bytes of workspace, more than the " +
+                                                      std::to_string(ws_bytes) + " given");
+    }
     uint32_t* keys = (uint32_t*)ws;
     uint32_t* keys2 = (uint32_t*)(ws + vec);
     uint32_t* idx = (uint32_t*)(ws + 2 * vec);
-    uint32_t* perm = (uint32_t*)(ws + 3 * vec);
+    uint32_t* idx2 = (uint32_t*)(ws + 3 * vec);
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255u) / 256u, (uint64_t)ctx->num_cus * 16u);
-    if (a.n_dev)
+    if (queue)
         hipLaunchKernelGGL(k_queue_keys, dim3(grid), dim3(256), 0, s, a.rays, a.n_dev, n, a.scene->frame, keys, idx);
     else
         hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(256), 0, s, a.rays, n, a.scene->frame, keys, idx);
     RT_HIP(ctx, hipGetLastError());
-    RT_HIP(ctx, rocprim::radix_sort_pairs((void*)(ws + 4 * vec), sort_bytes, keys, keys2, idx, perm, n, 0u,
-                                          key_bits, s));
-    *order = perm;
+    // double buffers: the sort ping-pongs between each pair, and the sorted
+    // ray indices end in whichever buffer holds the last pass
+    rocprim::double_buffer<uint32_t> kb(keys, keys2), vb(idx, idx2);
+    RT_HIP(ctx, rocprim::radix_sort_pairs((void*)(ws + 4 * vec), sort_bytes, kb, vb, n, 0u, order_key_bits(queue), s));
+    *order = vb.current();
     return RT_OK;
+}
+
+// The sorted order of a query's rays in ctx's order workspace, or null when
+// the batch keeps index order.
+int ray_order(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, const uint32_t** order) {
+    return ray_order_in(ctx, a, s, nullptr, 0, order);
 }
 
 int resolved_layout(const rt_ctx* ctx) {
@@ -312,6 +354,8 @@ int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s) {
     // layout leaves them out); the scene's arrays are immutable otherwise
     const uint32_t layout = (uint32_t)resolved_layout(ctx);
     if ((layout == RT_LAYOUT_F32 && !sc->d_pnodes) || (layout == RT_LAYOUT_Q16 && !sc->d_qnodes)) {
+        if (a.no_alloc)
+            return set_error(ctx, RT_ERR_UNSUPPORTED, "the scene's binary node records are not built yet");
         const int rc = scene_binary_records(ctx, const_cast<rt_scene*>(sc), layout);
         if (rc) return rc;
     }

@@ -184,7 +184,8 @@ _lib = None
 EXPORTS = [
     "rt_create", "rt_destroy", "rt_last_error", "rt_status_string", "rt_scene_upload", "rt_scene_info_get",
     "rt_scene_check", "rt_tuning_set", "rt_tuning_get",
-    "rt_scene_release", "rt_closest_hit", "rt_closest_hit_stats", "rt_render", "rt_render_frames",
+    "rt_scene_release", "rt_closest_hit", "rt_closest_hit_stats", "rt_closest_hit_noalloc", "rt_render",
+    "rt_render_frames",
     "rt_render_last_rays", "rt_render_set_timing", "rt_render_last_stats", "rt_query_last_stats", "rt_bvh_build", "rt_bvh_build_seeded", "rt_bvh_build_device", "rt_scene_lights",
     "rt_frame_loop_create", "rt_frame_loop_destroy", "rt_frame_loop_step", "rt_frame_loop_display",
     "rt_frame_loop_images", "rt_frame_loop_read", "rt_postprocess_gamma", "rt_gamma_table", "rt_image_write",
@@ -229,6 +230,7 @@ def lib():
         "rt_tuning_get": (i32, [P, ctypes.POINTER(Tuning)]),
         "rt_closest_hit": (i32, [P, P, P, u32, P, ctypes.POINTER(QueryParams), P]),
         "rt_closest_hit_stats": (i32, [P, P, P, u32, P, P, P, P]),
+        "rt_closest_hit_noalloc": (i32, [P, P, P, u32, P, ctypes.POINTER(QueryParams), P]),
         "rt_render": (i32, [P, P, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), P, P, u32, u32, P]),
         "rt_render_frames": (i32, [P, P, ctypes.POINTER(Ubo), ctypes.POINTER(RenderParams), u32, P, P, u32, u32, P]),
         "rt_render_last_rays": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
