@@ -38,9 +38,15 @@ Progress lines go to stderr; the JSON line is the only stdout.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import re
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -52,20 +58,180 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_packed_clusters4M_traffic.t
 CANON_SAMPLE = 65536
 
 
+# Host threads of the CPU baseline: the cores this process may run on,
+# capped at the 16-core host share a GPU has on the MI355X boxes it runs on.
+CPU_SHARE = 16
+# Shader engines of the MI355X (8 XCDs x 4): SQ_BUSY_CYCLES sums over them.
+SHADER_ENGINES = 32
+
+
+def visible_cores() -> int:
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
 def host_threads() -> int:
-    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    return max(1, min(16, n))
+    return max(1, min(CPU_SHARE, visible_cores()))
+
+
+def kernel_sources_sha() -> str:
+    """Hash of the traversal kernel's sources, to tell whether a committed
+    counter figure was taken on the kernel this tree builds."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("rt_trace_dev.hpp", "rt_trace.hip"):
+        with open(os.path.join(ROOT, "vulkan-compute-ray-tracing_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def read_traffic():
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE)."""
+    """L2-to-memory bytes per launch from the committed counter profile, and
+    whether that profile was taken on the kernel sources of this tree."""
+    vals = {}
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("bytes_per_launch="):
-                return float(line.split("=", 1)[1])
+            if "=" in line and not line.startswith("#"):
+                k, v = line.strip().split("=", 1)
+                vals[k] = v
     except OSError:
-        pass
-    return None
+        return None, None
+    if "bytes_per_launch" not in vals:
+        return None, None
+    return float(vals["bytes_per_launch"]), vals.get("kernel_sources_sha") == kernel_sources_sha()
+
+
+# Counter passes (rocprofv3 takes at most 4 TCC and 8 SQ counters per run).
+# L2-to-memory bytes: every 32-byte unit a TCC read or write request sends
+# to DRAM (a 64-byte request counts 2, a 128-byte one 4) -- what leaves the
+# L2 for HBM, which the Infinity Cache may still serve.
+PMC_PASSES = (
+    ("TCC_EA0_RDREQ_DRAM_32B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum", "TCC_EA0_WRREQ_ATOMIC_DRAM_32B_sum",
+     "TCC_HIT_sum"),
+    ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_BUSY_CYCLES", "SQ_ACTIVE_INST_VALU",
+     "SQ_THREAD_CYCLES_VALU", "TCC_MISS_sum"),
+)
+# kernel classes counted: the timed query kernel (not the counting variant),
+# the render's per-bounce traversals and its shading kernel
+KERNEL_CLASSES = (("query", re.compile(r"k_trace_packed<\d+, \d+, \d+, false|k_trace_packedILi\d+ELi\d+ELi\d+ELb0")),
+                  ("render_trace", re.compile(r"k_trace_render")), ("shade", re.compile(r"k_shade")),
+                  ("copy", re.compile(r"copyBuffer")))
+# the known-size copy the counter workload ends with: it calibrates the
+# byte counts (bytes read and written by a plain device-to-device copy)
+CALIBRATION_MIB = 1024
+
+
+def add_counter_rows(agg: dict, rows, k: int):
+    """Adds one counter pass (rows of rocprofv3's counter_collection.csv) to
+    agg[class]: counters summed over the class's dispatches, the dispatch
+    count of pass k, and (pass 0) each dispatch's duration."""
+    # per dispatch: its kernel class, grid size and counter values
+    per = {}
+    for row in rows:
+        name = row.get("Kernel_Name", "")
+        cls = next((c for c, rx in KERNEL_CLASSES if rx.search(name)), None)
+        if cls is None:
+            continue
+        key = row.get("Dispatch_Id") or row.get("Correlation_Id")
+        dd = per.setdefault((cls, key), {"grid": float(row.get("Grid_Size", 0) or 0), "c": {}, "ns": None})
+        dd["c"][row.get("Counter_Name", "")] = float(row.get("Counter_Value", 0) or 0)
+        if row.get("Start_Timestamp") and row.get("End_Timestamp"):
+            dd["ns"] = float(row["End_Timestamp"]) - float(row["Start_Timestamp"])
+    # the calibration copy is the class's largest dispatch (other copies are small transfers)
+    big_copy = max((key for (cls, key) in per if cls == "copy"), key=lambda kk: per[("copy", kk)]["grid"],
+                   default=None)
+    for (cls, key), dd in per.items():
+        if cls == "copy" and key != big_copy:
+            continue
+        a = agg.setdefault(cls, {})
+        for ctr, v in dd["c"].items():
+            a[ctr] = a.get(ctr, 0.0) + v
+        a[f"_dispatches{k}"] = a.get(f"_dispatches{k}", 0) + 1
+        if k == 0 and dd["ns"] is not None:
+            a.setdefault("_ns", {})[key] = dd["ns"]
+
+
+def run_counter_passes(args) -> dict | None:
+    """Runs this script's counter workload (--pmc-child: the timed query,
+    then the render unless --no-render) under rocprofv3 once per counter
+    pass, and sums each counter over each kernel class's dispatches."""
+    if not shutil.which("rocprofv3"):
+        log("counters: rocprofv3 not found")
+        return None
+    agg = {}
+    for k, counters in enumerate(PMC_PASSES):
+        d = tempfile.mkdtemp(prefix="rtamd_pmc_")
+        cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc", *counters, "-f", "csv", "-d", d, "-o", "pmc",
+               "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--tris", str(args.tris),
+               "--rays", str(args.rays), "--layout", args.layout, "--refill", str(args.refill)]
+        if args.no_render:
+            cmd.append("--no-render")
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        rows = []
+        for f in files:
+            with open(f, newline="") as fh:
+                rows.extend(csv.DictReader(fh))
+        shutil.rmtree(d, ignore_errors=True)
+        # rocprofv3 may end with a signal after writing whole results: rows decide
+        if not rows:
+            log(f"counters: pass {k} wrote no rows (exit {r.returncode}): {r.stderr.strip()[-300:]}")
+            return None
+        add_counter_rows(agg, rows, k)
+        log(f"counters: pass {k} ({len(counters)} counters) in {time.perf_counter() - t0:.1f} s")
+    return agg
+
+
+def counter_summary(c: dict, num_cus: int) -> dict:
+    """Per-dispatch means of one kernel class and the figures they give."""
+    d0, d1 = c.get("_dispatches0", 0), c.get("_dispatches1", 0)
+    if not d0 or not d1:
+        return {}
+    g = lambda k, d: c.get(k, 0.0) / d
+    rd = 32.0 * g("TCC_EA0_RDREQ_DRAM_32B_sum", d0)
+    wr = 32.0 * (g("TCC_EA0_WRREQ_WRITE_DRAM_32B_sum", d0) + g("TCC_EA0_WRREQ_ATOMIC_DRAM_32B_sum", d0))
+    hit, miss = g("TCC_HIT_sum", d0), g("TCC_MISS_sum", d1)
+    act, busy = g("SQ_ACTIVE_INST_VALU", d1), g("SQ_BUSY_CYCLES", d1)
+    simds = 4 * num_cus
+    out = {"dispatches": d0, "dram_read_bytes": rd, "dram_write_bytes": wr, "dram_bytes": rd + wr,
+           "l2_hit": hit / (hit + miss) if hit + miss else None,
+           "valu_insts": g("SQ_INSTS_VALU", d1), "salu_insts": g("SQ_INSTS_SALU", d1), "waves": g("SQ_WAVES", d1),
+           # a wave64 VALU instruction holds a 16-lane SIMD for 4 cycles; SQ_BUSY_CYCLES sums the shader engines
+           "valu_busy": act * 4.0 / (simds * busy / SHADER_ENGINES) if busy else None,
+           "valu_lanes_active": g("SQ_THREAD_CYCLES_VALU", d1) / (64.0 * act) if act else None}
+    ns = c.get("_ns", {})
+    if ns:
+        out["profiled_kernel_ms"] = sum(ns.values()) / len(ns) / 1e6
+    return out
+
+
+def pmc_child(args, threads: int):
+    """The workload the counter passes profile: the timed query (one warm-up
+    and two launches) and, unless --no-render, one 1080p 4-frame render."""
+    import torch
+    import rtamd
+    ctx = rtamd.Context(0)
+    layout = {"w4": rtamd.LAYOUT_W4, "q16": rtamd.LAYOUT_Q16, "f32": rtamd.LAYOUT_F32}[args.layout]
+    ctx.set_tuning(node_layout=layout, refill=args.refill)
+    hs = rtamd.scene_clusters(args.tris, 3, threads=threads)
+    scene = ctx.upload(hs)
+    rays = rtamd.rays_to_device(rtamd.rays_box(args.rays, 4, 4.0))
+    hits = torch.empty((rays.shape[0], 4), dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        ctx.closest_hit(scene, rays, hits)
+    torch.cuda.synchronize()
+    if not args.no_render:
+        tgt = torch.zeros((1080, 1920, 4), dtype=torch.uint8, device="cuda")
+        acc = torch.zeros_like(tgt)
+        ctx.render_frames(scene, [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8, target=tgt, accum=acc)
+        torch.cuda.synchronize()
+    scene.release()
+    ctx.close()
+    src = torch.ones(CALIBRATION_MIB << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def log(msg: str):
@@ -254,6 +420,9 @@ def cpu_baseline(hs, rays_np, seconds: float, threads: int):
         O.closest_hit(isc, rr, O.TRAV_REFERENCE, threads=threads)
         medium[f"{name}_mrays_per_s"] = round(len(rr) / (time.perf_counter() - t0) / 1e6, 4)
     return {"value": round(m / dt / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "host_cores_visible": visible_cores(),
+            "cores_note": f"threads = the cores this process may use, capped at the {CPU_SHARE}-core host share "
+                          "one GPU has on this machine",
             "sample": f"first {m} of the {len(rays_np)} incoherent rays, oracle hit_bvh in the shader's "
                       f"visiting order (line test, both children pushed), {threads} threads, median of 5 runs "
                       f"of {dt:.1f} s after a warm-up; CPU restatement, not lavapipe",
@@ -302,13 +471,26 @@ def bench_device_build(ctx, hs, rays, threads: int, layout: int, node_bytes: int
 
 
 def cpu_model() -> str:
+    """The first processor's identity as /proc/cpuinfo gives it: model name,
+    family / model / stepping numbers and clock (a virtual machine may report
+    a shortened model name, so the numbers are kept beside it)."""
+    fields = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if not line.strip():
+                break
+            if ":" in line:
+                k, v = line.split(":", 1)
+                fields.setdefault(k.strip(), v.strip())
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    parts = [fields.get("model name", "unknown")]
+    ids = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    if ids:
+        parts.append(", ".join(ids))
+    if "cpu MHz" in fields:
+        parts.append(f"{float(fields['cpu MHz']):.0f} MHz")
+    return "; ".join(parts)
 
 
 def copy_bandwidth(mib: int = 1024) -> float:
@@ -438,8 +620,15 @@ def main():
     ap.add_argument("--refill", type=int, default=0, help="idle lanes before a wave refetches (0 = library default)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the measurement, copy a buffer of this many MiB once (a known-size kernel for "
-                         "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
+                         "checking the memory-traffic counters)")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the counter passes (rocprofv3 --pmc runs of the query and render) and report the "
+                         "committed profile's traffic instead")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args, host_threads())
+        return
     if args.plain_only:
         args.no_render = args.no_batched = args.no_served = True
     if args.gpus != 1:
@@ -507,8 +696,7 @@ def main():
     order_ms = sum(st.order_ms for st in stages) / len(stages)
     sorted_rays = bool(stages[0].sorted)
     value = n / (ms_per_step / 1e3) / 1e6
-    achieved = bytes_per_launch / (kernel_ms / 1e3) / 1e9
-    traffic = read_traffic()
+    record_gbs = bytes_per_launch / (kernel_ms / 1e3) / 1e9
     copy_gbs = copy_bandwidth()
 
     out = {
@@ -521,16 +709,14 @@ def main():
                    "traversal": f"ordered, persistent, {args.layout} {node_bytes}-byte node records",
                    "trace_order": "origin Morton order (sorted on the device inside the timed call)"
                    if sorted_rays else "index order"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_trace_packed", "kernel_ms": round(kernel_ms, 4),
+        "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                     "traffic": None, "kernel": "k_trace_packed", "kernel_ms": round(kernel_ms, 4),
                      "call_ms": round(call_ms, 4), "order_ms": round(order_ms, 4),
-                     "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "stream_copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
-                     # measured HBM bytes (the traffic figure) over the same kernel time: the
-                     # algorithmic bytes count every record fetch, about half of which hit L2
-                     "hbm_gbs": round(traffic / (kernel_ms / 1e3) / 1e9, 1) if traffic else None,
-                     "hbm_frac": round(traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None},
+                     # every node and triangle record fetch priced at its record size (hits in
+                     # the caches included), per launch and over the kernel's time
+                     "record_fetch_bytes_per_launch": bytes_per_launch, "record_fetch_gbs": round(record_gbs, 1),
+                     "record_fetch_frac": round(record_gbs / HBM_PEAK_GBS, 4),
+                     "stream_copy_gbs": round(copy_gbs, 1)},
         "visits": {"node_records_per_ray": round(v_node / n, 2), "tri_records_per_ray": round(v_tri / n, 3)},
     }
 
@@ -584,6 +770,71 @@ def main():
             log(f"render on the device-built tree: {out['render']['device_tree_ms']} ms, "
                 f"rmse {out['render']['device_tree_rmse']} against the given tree's image")
             dscene.release()
+
+    # counters of the timed kernels (and the render's), in child runs under rocprofv3
+    agg = None if args.no_pmc else run_counter_passes(args)
+    num_cus = torch.cuda.get_device_properties(0).multi_processor_count
+    rf = out["roofline"]
+    q = counter_summary(agg["query"], num_cus) if agg and "query" in agg else {}
+    cal = counter_summary(agg["copy"], num_cus) if agg and "copy" in agg else {}
+    cal_read = cal_write = 1.0
+    if cal and cal["dram_read_bytes"] > 0 and cal["dram_write_bytes"] > 0:
+        # a copy of known size reads and writes CALIBRATION_MIB MiB: scale the counts by what it shows
+        known = float(CALIBRATION_MIB << 20)
+        cal_read, cal_write = known / cal["dram_read_bytes"], known / cal["dram_write_bytes"]
+        rf["traffic_calibration"] = {"copy_mib": CALIBRATION_MIB, "read_factor": round(cal_read, 4),
+                                     "write_factor": round(cal_write, 4)}
+    if q:
+        q["dram_bytes"] = q["dram_read_bytes"] * cal_read + q["dram_write_bytes"] * cal_write
+    if q:
+        traffic = q["dram_bytes"]
+        rf["traffic_source"] = ("measured in this run: rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B_sum, "
+                                "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum, TCC_EA0_WRREQ_ATOMIC_DRAM_32B_sum (x 32 bytes), "
+                                f"mean of {q['dispatches']} launches")
+    else:
+        traffic, current = read_traffic()
+        rf["traffic_source"] = (f"{os.path.relpath(TRAFFIC_FILE, ROOT)} (committed profile; "
+                                + ("taken on these kernel sources)" if current else "STALE: the kernel sources "
+                                   "changed since it was taken)")) if traffic else "none"
+    if traffic:
+        achieved = traffic / (kernel_ms / 1e3) / 1e9
+        rf.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                   "traffic": int(traffic), "traffic_vs_record_fetch": round(traffic / bytes_per_launch, 4)})
+    if q:
+        rf["l2_hit"] = round(q["l2_hit"], 4) if q["l2_hit"] is not None else None
+        if "profiled_kernel_ms" in q:
+            rf["profiled_kernel_ms"] = round(q["profiled_kernel_ms"], 4)
+        # what binds the loop: VALU issue, not bandwidth
+        rf["valu"] = {"busy": round(q["valu_busy"], 4) if q["valu_busy"] else None,
+                      "lanes_active": round(q["valu_lanes_active"], 4) if q["valu_lanes_active"] else None,
+                      "wave_insts_per_ray": round(q["valu_insts"] / n, 1),
+                      "salu_insts_per_ray": round(q["salu_insts"] / n, 1),
+                      "wave_insts_per_record": round(q["valu_insts"] / max(1, v_node + v_tri), 2)}
+        rf["binding_resource"] = "valu issue" if (q["valu_busy"] or 0) > 0.8 else "memory latency"
+    if agg and "render" in out:
+        r = out["render"]
+        rays_r = r["rays"]
+        tr = counter_summary(agg.get("render_trace", {}), num_cus)
+        sh = counter_summary(agg.get("shade", {}), num_cus)
+        for cs in (tr, sh):
+            if cs:
+                cs["dram_bytes"] = cs["dram_read_bytes"] * cal_read + cs["dram_write_bytes"] * cal_write
+        rc = {}
+        for name, cs, ms in (("trace", tr, r["trace_primary_ms"] + r["trace_secondary_ms"]), ("shade", sh, r["shade_ms"])):
+            if not cs:
+                continue
+            tot = lambda key: cs[key] * cs["dispatches"]
+            rc[name] = {"dispatches": cs["dispatches"], "dram_bytes": int(tot("dram_bytes")),
+                        "dram_bytes_per_ray": round(tot("dram_bytes") / rays_r, 1),
+                        "dram_gbs": round(tot("dram_bytes") / (ms / 1e3) / 1e9, 1),
+                        "dram_frac": round(tot("dram_bytes") / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "l2_hit": round(cs["l2_hit"], 4) if cs["l2_hit"] is not None else None,
+                        "valu_busy": round(cs["valu_busy"], 4) if cs["valu_busy"] else None,
+                        "valu_lanes_active": round(cs["valu_lanes_active"], 4) if cs["valu_lanes_active"] else None,
+                        "valu_wave_insts_per_ray": round(tot("valu_insts") / rays_r, 1)}
+        if rc:
+            r["counters"] = dict(rc, source="measured in this run (rocprofv3 --pmc, one render), per ray of the "
+                                            "render's primary + secondary rays, over the stage times above")
 
     if not args.plain_only:
         out["device_build"] = bench_device_build(ctx, hs, rays, threads, layout, node_bytes)
