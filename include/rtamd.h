@@ -246,8 +246,9 @@ int rt_tuning_get(const rt_ctx* ctx, rt_tuning* t);
 
 /* ----------------------------------------------------------------- batched */
 /* Many (scene, ray batch) items traced by ONE persistent launch with a
- * global work queue, optionally regrouping each item's rays by direction
- * octant and origin cell first, replayed from a hipGraph.  Each item gives
+ * global work queue, optionally regrouping each item's rays first (by
+ * origin cell, or by direction octant and origin cell), replayed from a
+ * hipGraph.  Each item gives
  * the same hits as rt_closest_hit on it alone (triangles only; the scenes
  * must take the packed path).  No counterpart in the reference, which
  * dispatches one image of one scene per frame (main.cpp:228). */
@@ -261,28 +262,37 @@ typedef struct rt_batch_item {
 
 typedef struct rt_batch_config {
     uint32_t block;          /* threads per work-group: 128 or 256 (0 = 256)       */
-    uint32_t stack_lds;      /* 0 = whole stack in LDS; 8, 16 or 32 = top entries   */
-    uint32_t regroup;        /* 1 = bin each item's rays by octant and origin cell  */
+    uint32_t stack_lds;      /* 0 = whole stack in LDS; 8, 16 or 32 = top entries
+                                (4-wide records only)                             */
+    uint32_t regroup;        /* 0 = rays in the caller's order; 1 = bin each item's
+                                rays by origin cell (16^3, Morton order); 2 = by
+                                direction octant, then origin cell (8^3)          */
     uint32_t refill;         /* idle lanes (1..64) before a wave fetches rays       */
     uint32_t blocks_per_cu;  /* 0 = occupancy limit                                 */
     uint32_t use_graph;      /* 1 = replay the launch sequence from a hipGraph      */
-    uint32_t reserved[2];
+    uint32_t regroup_interval; /* rays binned together: 0 = each whole item, or a
+                                power of two >= 65536 = windows of that many
+                                consecutive rays of an item                       */
+    uint32_t node_layout;    /* node records read: RT_LAYOUT_W4 (or 0), 4-wide
+                                64-byte records; RT_LAYOUT_Q16, binary 32-byte
+                                records (built for the scenes on first use)       */
 } rt_batch_config;
 
 typedef struct rt_batch rt_batch;
 
 /* Builds the device item table and work queue for `n` items (their ray and
  * hit buffers are the caller's) and captures the launch sequence.  NULL
- * config = defaults (256, 0, 0, 32, 0, 1). */
+ * config = defaults (256, 0, 0, 32, 0, 1, 0, RT_LAYOUT_W4). */
 int rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, uint32_t n, const rt_batch_config* config,
                     rt_batch** out);
 int rt_batch_set_config(rt_ctx* ctx, rt_batch* batch, const rt_batch_config* config);
 int rt_batch_get_config(const rt_batch* batch, rt_batch_config* config);
 /* Enqueues the whole batch on `stream`. */
 int rt_batch_launch(rt_ctx* ctx, rt_batch* batch, void* stream);
-/* Times candidate configurations on this batch (coordinate descent over
- * block, stack_lds, regroup, refill and blocks_per_cu; `trials` timed launches
- * each), applies the fastest and returns it.  With a cache file, a stored
+/* Times candidate configurations on this batch (coordinate descent, two
+ * passes, over block, node_layout, stack_lds, regroup, regroup_interval,
+ * refill and blocks_per_cu; `trials` timed launches each), applies the
+ * fastest and returns it.  With a cache file, a stored
  * result for the same shape (item count, log2 rays, log2 triangles) is reused
  * and new results are appended. */
 int rt_batch_autotune(rt_ctx* ctx, rt_batch* batch, uint32_t trials, const char* cache_path,

@@ -51,6 +51,9 @@ def check(rt, items, refs):
     dict(stack_lds=16, regroup=1),
     dict(block=128, refill=16),
     dict(blocks_per_cu=2, refill=64),
+    dict(regroup=2),
+    dict(node_layout=2),
+    dict(node_layout=2, block=128, regroup=2, regroup_interval=1 << 16),
 ])
 def test_batch_matches_plain(rt, gpu_ctx, mixed, cfg):
     _, items, refs = mixed
@@ -145,3 +148,41 @@ def test_batch_drain_sharing_on_ties(rt, gpu_ctx, steal):
         gpu_ctx.set_tuning()
     grid.release()
     soup.release()
+
+
+def test_batch_regroup_windows_and_layouts(rt, gpu_ctx):
+    """Regrouping within windows of consecutive rays (an item spans several
+    windows) and the binary 16-bit records: hits equal the plain kernel's."""
+    scenes = [gpu_ctx.upload(rt.scene_clusters(n, 500 + i)) for i, n in enumerate((60000, 2000))]
+    counts = [300001, 70000, 131072]
+    items, refs = [], []
+    for k, c in enumerate(counts):
+        rays = rt.rays_to_device(rt.rays_box(c, 600 + k, 4.0))
+        items.append((scenes[k % 2], rays, torch.full((c, 4), -7.0, device="cuda")))
+        refs.append(rt.hits_from_device(gpu_ctx.closest_hit(scenes[k % 2], rays)))
+    for cfg in (dict(regroup=1, regroup_interval=1 << 16), dict(regroup=2, regroup_interval=1 << 17),
+                dict(regroup=2, regroup_interval=1 << 20), dict(regroup=1, node_layout=2, regroup_interval=1 << 16)):
+        print(f"case: windows {cfg}", flush=True)
+        for _, _, hits in items:
+            hits.fill_(-7.0)
+        b = rt.Batch(gpu_ctx, items, rt.batch_config(**cfg))
+        b.launch()
+        check(rt, items, refs)
+        # a change of configuration re-uploads the item table before the next capture
+        b.set_config(rt.batch_config(regroup=2, regroup_interval=1 << 16))
+        for _, _, hits in items:
+            hits.fill_(-7.0)
+        b.launch()
+        check(rt, items, refs)
+        b.destroy()
+    for sc in scenes:
+        sc.release()
+
+
+def test_batch_rejects_invalid_configs(rt, gpu_ctx, mixed):
+    _, items, _ = mixed
+    for cfg in (dict(regroup=3), dict(regroup=1, regroup_interval=1000), dict(regroup=1, regroup_interval=1 << 15),
+                dict(node_layout=2, stack_lds=16), dict(node_layout=1)):
+        print(f"case: invalid batch config {cfg}", flush=True)
+        with pytest.raises(rt.RtError):
+            rt.Batch(gpu_ctx, items, rt.batch_config(**cfg))

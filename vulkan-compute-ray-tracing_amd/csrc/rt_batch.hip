@@ -5,10 +5,13 @@
 // Work queue: the items' rays are numbered 0..total-1 item after item; a
 // lane that needs work takes the next numbers from one cursor (the same
 // per-lane refetch as the plain kernel) and finds its item through a
-// 64-ray chunk table.  Regrouping (optional) first bins each item's rays by
-// the 16x16x16 origin cell (Morton order) in the scene's box frame, so
-// neighbouring lanes start in the same region and share node fetches; the
+// 64-ray chunk table.  Regrouping (optional) first bins each item's rays --
+// by the 16x16x16 origin cell (Morton order) in the scene's box frame, or by
+// direction octant and then 8x8x8 origin cell -- within windows of a given
+// number of consecutive rays (the regroup interval; the whole item by
+// default), so neighbouring lanes start alike and share node fetches; the
 // binning permutation only changes which lane traces a ray, never a result.
+// The node records read are the 4-wide ones or the binary 16-bit ones.
 // Every step is a kernel (counters are cleared by a fill kernel, not a
 // memset), so the sequence can be captured into a graph and replayed.
 #include <algorithm>
@@ -28,8 +31,9 @@
 namespace rtamd {
 namespace {
 
-constexpr uint32_t kBins = 4096;         // 16^3 origin cells per item, Morton order
+constexpr uint32_t kBins = 4096;         // bins per regroup window: 16^3 origin cells, or 8 octants x 8^3 cells
 constexpr uint32_t kChunk = 64;          // rays per chunk-table entry
+constexpr uint32_t kMinInterval = 1u << 16;   // smallest regroup window (bounds the histogram)
 
 __device__ __forceinline__ uint32_t item_of(const BatchIo& bio, uint32_t g) {
     uint32_t b = bio.chunk_item[g / kChunk];
@@ -37,35 +41,45 @@ __device__ __forceinline__ uint32_t item_of(const BatchIo& bio, uint32_t g) {
     return b;
 }
 
-// Bin of every ray, and its rank inside the bin (order of the atomics).
-__global__ __launch_bounds__(256) void k_regroup_bin(BatchIo bio, uint32_t total, uint32_t* __restrict__ hist,
-                                                     uint32_t* __restrict__ bin_rank) {
+// Bin of every ray in its window, and its rank inside the bin (order of the
+// atomics).  mode 1: 16^3 origin cells; mode 2: direction octant (3 high
+// bits) over 8^3 origin cells.  Window w of an item holds its local rays
+// [w << shift, (w + 1) << shift); shift 31 makes the whole item one window.
+__global__ __launch_bounds__(256) void k_regroup_bin(BatchIo bio, uint32_t total, uint32_t mode, uint32_t shift,
+                                                     uint32_t* __restrict__ hist, uint32_t* __restrict__ bin_rank) {
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
         const uint32_t b = item_of(bio, g);
         const BatchEntry& e = bio.table[b];
         const uint32_t local = g - e.base;
-        const float4 o = reinterpret_cast<const float4*>(e.rays)[2 * (size_t)local];
-        // (v - org) * rs spans [2, 65002] over the item's scene; 16 cells per axis
-        auto cell = [](float v, float org, float rs) -> uint32_t {
-            const float q = (v - org) * rs * (16.0f / 65000.0f);
-            return q >= 15.0f ? 15u : (q > 0.0f ? (uint32_t)q : 0u);
+        const float4* rp = reinterpret_cast<const float4*>(e.rays) + 2 * (size_t)local;
+        const float4 o = rp[0];
+        // (v - org) * rs spans [2, 65002] over the item's scene
+        const float cells = mode == 2 ? 8.0f : 16.0f;
+        auto cell = [cells](float v, float org, float rs) -> uint32_t {
+            const float q = (v - org) * rs * (cells / 65000.0f);
+            return q >= cells - 1.0f ? (uint32_t)cells - 1u : (q > 0.0f ? (uint32_t)q : 0u);
         };
         auto spread4 = [](uint32_t x) -> uint32_t {   // bit k -> bit 3k
             return (x & 1u) | ((x & 2u) << 2) | ((x & 4u) << 4) | ((x & 8u) << 6);
         };
-        const uint32_t bin = (spread4(cell(o.x, e.frame.ox, e.frame.rx)) << 2) |
-                             (spread4(cell(o.y, e.frame.oy, e.frame.ry)) << 1) | spread4(cell(o.z, e.frame.oz, e.frame.rz));
-        uint32_t rank = atomicAdd(&hist[b * kBins + bin], 1u);
+        uint32_t bin = (spread4(cell(o.x, e.frame.ox, e.frame.rx)) << 2) |
+                       (spread4(cell(o.y, e.frame.oy, e.frame.ry)) << 1) | spread4(cell(o.z, e.frame.oz, e.frame.rz));
+        if (mode == 2) {
+            const float4 d = rp[1];
+            bin |= ((d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u)) << 9;
+        }
+        const uint32_t win = e.win0 + (local >> shift);
+        uint32_t rank = atomicAdd(&hist[(size_t)win * kBins + bin], 1u);
         bin_rank[2 * (size_t)g] = bin;
         bin_rank[2 * (size_t)g + 1] = rank;
     }
 }
 
-// Exclusive scan of each item's bin counts (one block per item).
-__global__ __launch_bounds__(256) void k_regroup_scan(uint32_t* __restrict__ hist, uint32_t n_items) {
+// Exclusive scan of each window's bin counts (one block per window).
+__global__ __launch_bounds__(256) void k_regroup_scan(uint32_t* __restrict__ hist, uint32_t n_windows) {
     __shared__ uint32_t part[256];
     const uint32_t b = blockIdx.x;
-    if (b >= n_items) return;
+    if (b >= n_windows) return;
     uint32_t* h = hist + (size_t)b * kBins;
     const uint32_t t = threadIdx.x;
     constexpr uint32_t kPer = kBins / 256;      // consecutive bins per thread
@@ -90,22 +104,28 @@ __global__ __launch_bounds__(256) void k_regroup_scan(uint32_t* __restrict__ his
     }
 }
 
-__global__ __launch_bounds__(256) void k_regroup_scatter(BatchIo bio, uint32_t total, const uint32_t* __restrict__ hist,
+// Each ray to its place: the item's base, its window's first ray, the
+// bin's offset inside the window and the ray's rank inside the bin.
+__global__ __launch_bounds__(256) void k_regroup_scatter(BatchIo bio, uint32_t total, uint32_t shift,
+                                                         const uint32_t* __restrict__ hist,
                                                          const uint32_t* __restrict__ bin_rank,
                                                          uint32_t* __restrict__ perm) {
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
         const uint32_t b = item_of(bio, g);
-        const uint32_t base = bio.table[b].base;
+        const BatchEntry& e = bio.table[b];
+        const uint32_t local = g - e.base;
+        const uint32_t w = local >> shift;
+        const uint32_t first = shift >= 31 ? 0u : w << shift;
         const uint32_t bin = bin_rank[2 * (size_t)g], rank = bin_rank[2 * (size_t)g + 1];
-        perm[base + hist[b * kBins + bin] + rank] = g - base;
+        perm[e.base + first + hist[(size_t)(e.win0 + w) * kBins + bin] + rank] = local;
     }
 }
 
-template <int BLOCK, int STACK, int RING>
+template <int BLOCK, int STACK, int RING, int LAYOUT>
 __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceIo io, BatchIo bio, int32_t* spill) {
     __shared__ int32_t stk[(RING ? RING : STACK) * BLOCK];
     TraceScene sc{};
-    trace_persistent<BLOCK, STACK, kLayoutW4, false, true, RING>(sc, io, bio, stk, spill);
+    trace_persistent<BLOCK, STACK, LAYOUT, false, true, RING>(sc, io, bio, stk, spill);
 }
 
 __global__ void k_fill(uint32_t* p, uint32_t value, uint32_t count) {
@@ -120,12 +140,28 @@ rt_batch_config default_config() {
     c.refill = 32;
     c.blocks_per_cu = 0;
     c.use_graph = 1;
+    c.regroup_interval = 0;
+    c.node_layout = RT_LAYOUT_W4;
     return c;
 }
 
+bool binary_layout(const rt_batch_config& c) { return c.node_layout == RT_LAYOUT_Q16; }
+
 bool valid_config(const rt_batch_config& c) {
+    const bool interval_ok = c.regroup_interval == 0 ||
+                             (c.regroup_interval >= kMinInterval && (c.regroup_interval & (c.regroup_interval - 1)) == 0);
+    const bool layout_ok = c.node_layout == 0 || c.node_layout == RT_LAYOUT_W4 || c.node_layout == RT_LAYOUT_Q16;
     return (c.block == 128 || c.block == 256) && (c.stack_lds == 0 || c.stack_lds == 8 || c.stack_lds == 16 || c.stack_lds == 32) &&
-           c.regroup <= 1 && c.refill >= 1 && c.refill <= 64 && c.blocks_per_cu <= 16 && c.use_graph <= 1;
+           c.regroup <= 2 && c.refill >= 1 && c.refill <= 64 && c.blocks_per_cu <= 16 && c.use_graph <= 1 && interval_ok &&
+           layout_ok && !(c.node_layout == RT_LAYOUT_Q16 && c.stack_lds != 0);
+}
+
+// log2 of the regroup window (31: the whole item)
+uint32_t interval_shift(const rt_batch_config& c) {
+    if (c.regroup_interval == 0) return 31u;
+    uint32_t k = 0;
+    while ((1u << k) < c.regroup_interval) ++k;
+    return k;
 }
 
 }  // namespace
@@ -149,6 +185,10 @@ struct rt_batch {
     int32_t* d_spill = nullptr;
     size_t spill_bytes = 0;
     rt_batch_config cfg{};
+    std::vector<BatchEntry> h_table;   // the item table as last uploaded
+    uint32_t hist_windows = 0;         // regroup windows the histogram holds (at the smallest interval)
+    uint32_t n_windows = 0;            // regroup windows under cfg's interval
+    uint32_t packed_height = 0;        // deepest binary tree among the items (binary records)
     uint32_t grid_blocks = 0;          // persistent blocks for cfg (set by a dry pass)
     bool dry = false;                  // size buffers only, launch nothing
     hipStream_t cap_stream = nullptr;
@@ -159,9 +199,9 @@ struct rt_batch {
 
 namespace {
 
-template <int BLOCK, int STACK, int RING>
+template <int BLOCK, int STACK, int RING, int LAYOUT>
 int launch_batch_kernel(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
-    auto kern = k_trace_batch<BLOCK, STACK, RING>;
+    auto kern = k_trace_batch<BLOCK, STACK, RING, LAYOUT>;
     if (b->dry) {
         // occupancy and the spill buffer are settled outside any capture
         int per_cu = 0;
@@ -194,14 +234,26 @@ int launch_batch_kernel(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
 
 template <int BLOCK, int STACK>
 int launch_batch_ring(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
-    if (b->cfg.stack_lds == 8 && STACK > 8) return launch_batch_kernel<BLOCK, STACK, 8>(ctx, b, s);
-    if (b->cfg.stack_lds == 16 && STACK > 16) return launch_batch_kernel<BLOCK, STACK, 16>(ctx, b, s);
-    if (b->cfg.stack_lds == 32 && STACK > 32) return launch_batch_kernel<BLOCK, STACK, 32>(ctx, b, s);
-    return launch_batch_kernel<BLOCK, STACK, 0>(ctx, b, s);
+    if (b->cfg.stack_lds == 8 && STACK > 8) return launch_batch_kernel<BLOCK, STACK, 8, kLayoutW4>(ctx, b, s);
+    if (b->cfg.stack_lds == 16 && STACK > 16) return launch_batch_kernel<BLOCK, STACK, 16, kLayoutW4>(ctx, b, s);
+    if (b->cfg.stack_lds == 32 && STACK > 32) return launch_batch_kernel<BLOCK, STACK, 32, kLayoutW4>(ctx, b, s);
+    return launch_batch_kernel<BLOCK, STACK, 0, kLayoutW4>(ctx, b, s);
+}
+
+// Binary 16-bit records: one deferred child per level, whole stack in LDS.
+template <int BLOCK>
+int launch_batch_binary(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
+    const uint32_t h = b->packed_height;
+    if (h <= 24) return launch_batch_kernel<BLOCK, 24, 0, kLayoutQ16>(ctx, b, s);
+    if (h <= 32) return launch_batch_kernel<BLOCK, 32, 0, kLayoutQ16>(ctx, b, s);
+    if (h <= 48) return launch_batch_kernel<BLOCK, 48, 0, kLayoutQ16>(ctx, b, s);
+    if (h <= 64) return launch_batch_kernel<BLOCK, 64, 0, kLayoutQ16>(ctx, b, s);
+    return set_error(ctx, RT_ERR_UNSUPPORTED, "batch: a binary tree deeper than 64 levels");
 }
 
 template <int BLOCK>
 int launch_batch_block(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
+    if (binary_layout(b->cfg)) return launch_batch_binary<BLOCK>(ctx, b, s);
     const uint32_t need = b->stack_need;
     if (need <= 24) return launch_batch_ring<BLOCK, 24>(ctx, b, s);
     if (need <= 36) return launch_batch_ring<BLOCK, 36>(ctx, b, s);
@@ -217,15 +269,16 @@ int launch_sequence(rt_ctx* ctx, rt_batch* b, hipStream_t s) {
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, b->d_cursor, 0u, kCursorBlock);
     RT_HIP(ctx, hipGetLastError());
     if (b->cfg.regroup) {
-        const uint32_t nh = (uint32_t)b->items.size() * kBins;
+        const uint32_t nh = b->n_windows * kBins;
+        const uint32_t shift = interval_shift(b->cfg);
         const uint32_t grid = std::max(1u, std::min((b->total + 255u) / 256u, (uint32_t)ctx->num_cus * 16u));
         BatchIo bio{b->d_table, b->d_offsets, b->d_chunk_item, nullptr};
         hipLaunchKernelGGL(k_fill, dim3(std::min((nh + 255u) / 256u, 1024u)), dim3(256), 0, s, b->d_hist, 0u, nh);
-        hipLaunchKernelGGL(k_regroup_bin, dim3(grid), dim3(256), 0, s, bio, b->total, b->d_hist, b->d_bin_rank);
-        hipLaunchKernelGGL(k_regroup_scan, dim3((uint32_t)b->items.size()), dim3(256), 0, s, b->d_hist,
-                           (uint32_t)b->items.size());
-        hipLaunchKernelGGL(k_regroup_scatter, dim3(grid), dim3(256), 0, s, bio, b->total, b->d_hist, b->d_bin_rank,
-                           b->d_perm);
+        hipLaunchKernelGGL(k_regroup_bin, dim3(grid), dim3(256), 0, s, bio, b->total, b->cfg.regroup, shift, b->d_hist,
+                           b->d_bin_rank);
+        hipLaunchKernelGGL(k_regroup_scan, dim3(b->n_windows), dim3(256), 0, s, b->d_hist, b->n_windows);
+        hipLaunchKernelGGL(k_regroup_scatter, dim3(grid), dim3(256), 0, s, bio, b->total, shift, b->d_hist,
+                           b->d_bin_rank, b->d_perm);
         RT_HIP(ctx, hipGetLastError());
     }
     return b->cfg.block == 128 ? launch_batch_block<128>(ctx, b, s) : launch_batch_block<256>(ctx, b, s);
@@ -238,10 +291,45 @@ void drop_graph(rt_batch* b) {
     b->graph = nullptr;
 }
 
-// Settles the grid and spill buffer for b->cfg, then captures the sequence
-// into a graph when the configuration asks for one.
+// The item table b->cfg reads: each item's first regroup window under the
+// configured interval, and the binary records when the configuration reads
+// them (built now for scenes uploaded without).  Uploaded when it changed,
+// after the device has finished the launches that read the old one.
+int prepare_table(rt_ctx* ctx, rt_batch* b) {
+    std::vector<BatchEntry> t = b->h_table;
+    const uint32_t shift = interval_shift(b->cfg);
+    uint32_t win = 0, hq = 0;
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        BatchEntry& e = t[i];
+        e.win0 = win;
+        win += shift >= 31 ? 1u : std::max(1u, (uint32_t)(((uint64_t)e.count + (1u << shift) - 1u) >> shift));
+        if (binary_layout(b->cfg)) {
+            rt_scene* sc = const_cast<rt_scene*>(b->items[i].scene);
+            if (!sc->d_qnodes) {
+                const int rc = scene_binary_records(ctx, sc, RT_LAYOUT_Q16);
+                if (rc) return rc;
+            }
+            e.qnodes = sc->d_qnodes;
+            e.root_q = sc->root_ref;
+            hq = std::max(hq, sc->packed_height);
+        }
+    }
+    if (win > b->hist_windows) return set_error(ctx, RT_ERR_INVALID, "batch: regroup interval below the smallest window");
+    b->n_windows = std::max(win, 1u);
+    b->packed_height = hq;
+    if (t.empty() || std::memcmp(t.data(), b->h_table.data(), sizeof(BatchEntry) * t.size()) == 0) return RT_OK;
+    RT_HIP(ctx, hipDeviceSynchronize());
+    RT_HIP(ctx, hipMemcpy(b->d_table, t.data(), sizeof(BatchEntry) * t.size(), hipMemcpyHostToDevice));
+    b->h_table = t;
+    return RT_OK;
+}
+
+// Settles the item table, grid and spill buffer for b->cfg, then captures
+// the sequence into a graph when the configuration asks for one.
 int capture(rt_ctx* ctx, rt_batch* b) {
     drop_graph(b);
+    int trc = prepare_table(ctx, b);
+    if (trc) return trc;
     b->tuning_gen = ctx->tuning_gen;
     b->dry = true;
     int rc = launch_sequence(ctx, b, nullptr);
@@ -285,7 +373,7 @@ std::string cache_key(const rt_batch* b) {
 
 template <int STACK>
 int launch_dyn_t(rt_ctx* ctx, const BatchIo& bio, uint32_t* cursor, uint32_t total, uint32_t refill, hipStream_t s) {
-    auto kern = k_trace_batch<256, STACK, 0>;
+    auto kern = k_trace_batch<256, STACK, 0, kLayoutW4>;
     int per_cu = 0;
     RT_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
     if (per_cu < 1) per_cu = 1;
@@ -360,13 +448,18 @@ extern "C" int rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, uint32_t
             e.tri_id = sc->d_ptri_id;
             e.rays = items[i].rays_dev;
             e.hits = items[i].hits_dev;
+            e.qnodes = sc->d_qnodes;
             e.frame = sc->frame;
             e.root = sc->root4;
+            e.root_q = sc->root_ref;
             e.base = offsets[i];
             e.count = items[i].count;
-            e.pad = 0;
+            e.win0 = i;
             offsets[i + 1] = offsets[i] + items[i].count;
+            b->hist_windows += std::max(1u, (uint32_t)(((uint64_t)items[i].count + kMinInterval - 1u) / kMinInterval));
         }
+        b->hist_windows = std::max(b->hist_windows, 1u);
+        b->h_table = table;
         const uint32_t chunks = (b->total + kChunk - 1) / kChunk;
         std::vector<uint32_t> chunk_item(std::max(chunks, 1u), 0);
         for (uint32_t i = 0, c = 0; i < n && c < chunks; ++i)
@@ -380,7 +473,7 @@ extern "C" int rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, uint32_t
         if ((rc = up((void**)&b->d_offsets, offsets.data(), sizeof(uint32_t) * offsets.size()))) break;
         if ((rc = up((void**)&b->d_chunk_item, chunk_item.data(), sizeof(uint32_t) * chunk_item.size()))) break;
         if ((rc = up((void**)&b->d_cursor, nullptr, sizeof(uint32_t) * kCursorBlock))) break;
-        if ((rc = up((void**)&b->d_hist, nullptr, sizeof(uint32_t) * kBins * std::max(n, 1u)))) break;
+        if ((rc = up((void**)&b->d_hist, nullptr, sizeof(uint32_t) * kBins * (size_t)b->hist_windows))) break;
         if ((rc = up((void**)&b->d_bin_rank, nullptr, sizeof(uint32_t) * 2 * std::max(b->total, 1u)))) break;
         if ((rc = up((void**)&b->d_perm, nullptr, sizeof(uint32_t) * std::max(b->total, 1u)))) break;
         if (hipStreamCreate(&b->cap_stream) != hipSuccess) { rc = set_error(ctx, RT_ERR_HIP, "hipStreamCreate"); break; }
@@ -461,6 +554,8 @@ extern "C" int rt_batch_autotune(rt_ctx* ctx, rt_batch* b, uint32_t trials, cons
                 else if (k == "regroup") c.regroup = v;
                 else if (k == "refill") c.refill = v;
                 else if (k == "blocks_per_cu") c.blocks_per_cu = v;
+                else if (k == "regroup_interval") c.regroup_interval = v;
+                else if (k == "node_layout") c.node_layout = v;
             }
             if (valid_config(c)) { found = c; hit = true; }
         }
@@ -500,17 +595,27 @@ extern "C" int rt_batch_autotune(rt_ctx* ctx, rt_batch* b, uint32_t trials, cons
     rt_batch_config cur = b->cfg;
     float cur_ms = 0.0f;
     int rc = time_cfg(cur, cur_ms);
-    const std::vector<uint32_t> blocks = {256, 128}, rings = {0, 32, 16, 8}, regroups = {0, 1},
+    // coordinate descent over: work-group size, node-fetch layout (4-wide or
+    // binary 16-bit records), LDS stack ring, regrouping (off, origin cell,
+    // direction octant + origin cell), regroup interval, refill threshold
+    // and blocks per CU
+    const std::vector<uint32_t> blocks = {256, 128}, layouts = {RT_LAYOUT_W4, RT_LAYOUT_Q16},
+                                rings = {0, 32, 16, 8}, regroups = {0, 1, 2}, intervals = {0, 1u << 16, 1u << 18},
                                 refills = {8, 16, 24, 32, 48, 64}, bpcs = {0, 2, 3, 4};
+    constexpr int kDims = 7;
     for (int pass = 0; pass < 2 && !rc; ++pass) {
-        for (int p = 0; p < 5 && !rc; ++p) {
-            const std::vector<uint32_t>& vals = p == 0 ? blocks : p == 1 ? rings : p == 2 ? regroups : p == 3 ? refills : bpcs;
+        for (int p = 0; p < kDims && !rc; ++p) {
+            const std::vector<uint32_t>& vals = p == 0 ? blocks : p == 1 ? layouts : p == 2 ? rings : p == 3 ? regroups
+                                              : p == 4 ? intervals : p == 5 ? refills : bpcs;
             for (uint32_t v : vals) {
                 rt_batch_config c = cur;
-                uint32_t* field = p == 0 ? &c.block : p == 1 ? &c.stack_lds : p == 2 ? &c.regroup
-                                : p == 3 ? &c.refill : &c.blocks_per_cu;
+                uint32_t* field = p == 0 ? &c.block : p == 1 ? &c.node_layout : p == 2 ? &c.stack_lds
+                                : p == 3 ? &c.regroup : p == 4 ? &c.regroup_interval : p == 5 ? &c.refill
+                                : &c.blocks_per_cu;
                 if (*field == v) continue;
                 *field = v;
+                // combinations that cannot run, or equal the current one (an interval without regrouping)
+                if (!valid_config(c) || (p == 4 && c.regroup == 0)) continue;
                 float ms = 0.0f;
                 if ((rc = time_cfg(c, ms))) break;
                 if (ms < cur_ms) { cur = c; cur_ms = ms; }
@@ -527,7 +632,9 @@ extern "C" int rt_batch_autotune(rt_ctx* ctx, rt_batch* b, uint32_t trials, cons
     if (cache_path && *cache_path) {
         std::ofstream o(cache_path, std::ios::app);
         o << key << " block=" << cur.block << " stack_lds=" << cur.stack_lds << " regroup=" << cur.regroup
-          << " refill=" << cur.refill << " blocks_per_cu=" << cur.blocks_per_cu << " ms=" << cur_ms << "\n";
+          << " refill=" << cur.refill << " blocks_per_cu=" << cur.blocks_per_cu
+          << " regroup_interval=" << cur.regroup_interval << " node_layout=" << cur.node_layout << " ms=" << cur_ms
+          << "\n";
     }
     return RT_OK;
 }

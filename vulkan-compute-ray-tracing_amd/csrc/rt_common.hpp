@@ -79,7 +79,8 @@ struct QFrame { float ox, oy, oz, sx, sy, sz, rx, ry, rz; };
 // allocation (rt_scene::tri_off), addressed by 32-bit offsets.
 enum { kLayoutF32 = 1, kLayoutQ16 = 2, kLayoutW4 = 3, kLayoutW4Joined = 4 };
 
-// One item of a batched launch: a resident scene's 4-wide tree and a ray
+// One item of a batched launch: a resident scene's packed tree (4-wide
+// records, and the binary 16-bit ones when the launch reads those) and a ray
 // batch with its hit buffer.  Global ray g of the launch belongs to the item
 // with base <= g < base + count.
 struct BatchEntry {
@@ -88,11 +89,13 @@ struct BatchEntry {
     const int32_t* tri_id;
     const rt_ray* rays;
     rt_hit* hits;
+    const PNodeQ* qnodes;         // binary 16-bit records, or null
     QFrame frame;
-    int32_t root;
+    int32_t root;                 // root of the 4-wide records
+    int32_t root_q;               // root of the binary records
     uint32_t base;
     uint32_t count;
-    uint32_t pad;
+    uint32_t win0;                // regrouping: the item's first window
 };
 struct BatchIo {
     const BatchEntry* table;

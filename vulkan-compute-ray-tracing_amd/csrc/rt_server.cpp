@@ -208,10 +208,12 @@ int launch_slot(rt_server* S, Slot& sl) {
             e.rays = sl.d_rays + off;
             e.hits = sl.d_hits + off;
             e.frame = sc->frame;
+            e.qnodes = nullptr;
             e.root = sc->root4;
+            e.root_q = sc->root_ref;
             e.base = off;
             e.count = r->n;
-            e.pad = 0;
+            e.win0 = 0;
             sl.h_offsets[i] = off;
             off += r->n;
             need = std::max(need, 3u * sc->height4 + 1u);

@@ -273,7 +273,8 @@ __device__ __forceinline__ float shfl_f(float x, uint32_t src) {
 template <int BLOCK, int STACK, int LAYOUT, bool STATS, bool BATCHED, int RING>
 __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const TraceIo& io, const BatchIo& bio,
                                                  int32_t* __restrict__ stk, int32_t* __restrict__ spill) {
-    static_assert(!BATCHED || LAYOUT == kLayoutW4, "batched launches use the 4-wide layout");
+    static_assert(!BATCHED || LAYOUT == kLayoutW4 || LAYOUT == kLayoutQ16,
+                  "batched launches read 4-wide or binary 16-bit records");
     constexpr bool kW4 = LAYOUT == kLayoutW4 || LAYOUT == kLayoutW4Joined;
     constexpr bool STEAL = kW4 && RING == 0 && !STATS;
     static_assert(RING == 0 || (RING & (RING - 1)) == 0, "ring size is a power of two");
@@ -334,6 +335,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     bool handoff = false;           // helper: done, result not yet handed over
     // the lane's scene (uniform unless BATCHED)
     const PNode4* l_w4 = sc.w4nodes;
+    const PNodeQ* l_q = sc.qnodes;
     const PTri* l_tris = sc.tris;
     const int32_t* l_triid = sc.tri_id;
     rt_hit* l_hits = io.hits;
@@ -464,7 +466,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                 float tn0, tn1;
                 int32_t c0, c1;
                 if (LAYOUT == kLayoutQ16) {
-                    const uint4* np = reinterpret_cast<const uint4*>(sc.qnodes + cur);
+                    const uint4* np = reinterpret_cast<const uint4*>(l_q + cur);
                     uint4 a = np[0], b = np[1];
                     h0 = slab(lo16(a.x), hi16(a.x), lo16(a.y), hi16(a.y), lo16(a.z), hi16(a.z), bo, inv,
                               tmin_rel, best_rel, tn0);
@@ -552,11 +554,12 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                         uint32_t b = bio.chunk_item[mine >> 6];
                         while (mine >= bio.offsets[b + 1]) ++b;
                         const BatchEntry* e = bio.table + b;
-                        l_w4 = e->nodes;
+                        if (kW4) l_w4 = e->nodes;
+                        else l_q = e->qnodes;
                         l_tris = e->tris;
                         l_triid = e->tri_id;
                         l_hits = e->hits;
-                        root = e->root;
+                        root = kW4 ? e->root : e->root_q;
                         q = e->frame;
                         ray_i = bio.perm ? bio.perm[mine] : mine - e->base;
                         r = load_ray_stream(e->rays, ray_i);

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "librtamd.so")
+# the in-tree build; RTAMD_LIB names another build of the same library
+LIB_PATH = os.environ.get("RTAMD_LIB") or os.path.join(PKG_DIR, "librtamd.so")
 
 RT_OK, RT_ERR_INVALID, RT_ERR_HIP, RT_ERR_NOMEM, RT_ERR_UNSUPPORTED, RT_ERR_BUSY, RT_ERR_CANCELLED, RT_ERR_NOT_FOUND = range(8)
 RT_MAT_LIGHT, RT_MAT_LAMBERTIAN, RT_MAT_METAL, RT_MAT_GLASS = range(4)
@@ -93,14 +94,15 @@ class BatchItem(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("stack_lds", ctypes.c_uint32), ("regroup", ctypes.c_uint32),
                 ("refill", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 2)]
+                ("regroup_interval", ctypes.c_uint32), ("node_layout", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
-def batch_config(block=256, stack_lds=0, regroup=0, refill=32, blocks_per_cu=0, use_graph=1) -> BatchConfig:
-    return BatchConfig(block, stack_lds, regroup, refill, blocks_per_cu, use_graph)
+def batch_config(block=256, stack_lds=0, regroup=0, refill=32, blocks_per_cu=0, use_graph=1, regroup_interval=0,
+                 node_layout=3) -> BatchConfig:
+    return BatchConfig(block, stack_lds, regroup, refill, blocks_per_cu, use_graph, regroup_interval, node_layout)
 
 
 class RenderStats(ctypes.Structure):
