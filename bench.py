@@ -612,9 +612,9 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--fifo-seconds", type=float, default=10.0)
-    ap.add_argument("--serve-rates", default="",
+    ap.add_argument("--serve-rates", default="8000",
                     help="comma-separated arrival rates (per s) for extra 6-second runs of the batching server "
-                         "(capacity beyond the 4000/s workload); off by default")
+                         "(capacity beyond the 4000/s workload); empty for none")
     ap.add_argument("--layout", choices=["w4", "q16", "f32"], default="w4",
                     help="node records: 4-wide 64-byte 16-bit (default), binary 32-byte 16-bit, binary 64-byte fp32")
     ap.add_argument("--refill", type=int, default=0, help="idle lanes before a wave refetches (0 = library default)")
