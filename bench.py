@@ -736,6 +736,11 @@ def main():
         ctx.render_frames(scene, [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8, target=tgt, accum=acc)
         st = ctx.last_render_stats()
         ctx.set_render_timing(False)
+        # the render's record fetches, from one untimed counted render
+        ctx.set_render_counting(True)
+        ctx.render_frames(scene, [0.0, 10.0, 0.0], 1920, 1080, 4, vfov=60.0, bounces=8, target=tgt, accum=acc)
+        r_nv, r_nt = ctx.last_render_counts()
+        ctx.set_render_counting(False)
         out["render"] = {"workload": "1920x1080, 4 spp (currentSample 0..3), 8 bounces, vfov 60, eye (0,0,10)",
                          "ms": round(rms, 3), "rays": prim + sec, "secondary_rays": sec,
                          "mrays_per_s": round((prim + sec) / (rms / 1e3) / 1e6, 2),
@@ -744,6 +749,13 @@ def main():
                          "shade_ms": round(st.shade_ms, 3),
                          "primary_trace_mrays_per_s": round(prim / st.trace_primary_ms / 1e3, 1),
                          "secondary_trace_mrays_per_s": round(sec / st.trace_secondary_ms / 1e3, 1)}
+        r_bytes = r_nv * node_bytes + r_nt * 48 + (prim + sec) * (32 + 16)
+        r_trace_s = (st.trace_primary_ms + st.trace_secondary_ms) / 1e3
+        out["render"].update({"node_records_per_ray": round(r_nv / (prim + sec), 2),
+                              "tri_records_per_ray": round(r_nt / (prim + sec), 3),
+                              "record_fetch_bytes": int(r_bytes),
+                              "record_fetch_gbs": round(r_bytes / r_trace_s / 1e9, 1),
+                              "record_fetch_frac": round(r_bytes / r_trace_s / 1e9 / HBM_PEAK_GBS, 4)})
         log(f"render: {rms:.1f} ms, secondary traversal {out['render']['secondary_trace_mrays_per_s']} Mrays/s")
         if not args.plain_only:
             # the same render on the device-built tree (RT_TREE_DEVICE): its

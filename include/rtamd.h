@@ -350,6 +350,13 @@ typedef struct rt_render_stats {
 } rt_render_stats;
 int rt_render_set_timing(rt_ctx* ctx, int enable);
 int rt_render_last_stats(rt_ctx* ctx, rt_render_stats* out);
+/* Record counting of renders (no counterpart in the reference): while on,
+ * a render's packed traversals run the counting variant (no drain work
+ * sharing, images unchanged) and rt_render_last_counts returns the node
+ * and triangle records its rays read in total -- the render's algorithmic
+ * bytes.  Waits for the render. */
+int rt_render_set_counting(rt_ctx* ctx, int enable);
+int rt_render_last_counts(rt_ctx* ctx, uint64_t* node_records, uint64_t* tri_records);
 typedef struct rt_query_stats {
     uint32_t timed, sorted;   /* sorted: the rays were traced in origin order */
     float order_ms, trace_ms, spheres_ms, total_ms;

@@ -161,3 +161,31 @@ def test_bench_render_against_the_oracle_on_sampled_rows(rt, gpu_ctx):
     want = want[::72]
     assert got[..., :3].mean() > 1.0
     assert rmse(got, want) <= 1e-3
+
+
+def test_render_record_counts(rt, gpu_ctx):
+    """A counted render gives the image of an uncounted one, and with one
+    bounce its record totals equal the counting query's over the same camera
+    rays (generated on the host with the shader's camera arithmetic)."""
+    hs = rt.scene_clusters(200000, 9, threads=16)
+    sc = gpu_ctx.upload(hs)
+    W, H, cam = 320, 240, [0.0, 10.0, 0.0]
+    print("case: counted render, 200k-triangle clustered soup, 320x240", flush=True)
+    img = gpu_ctx.render(sc, cam, W, H, vfov=60.0, bounces=4).cpu().numpy()
+    gpu_ctx.set_render_counting(True)
+    try:
+        counted = gpu_ctx.render(sc, cam, W, H, vfov=60.0, bounces=4).cpu().numpy()
+        nv4, nt4 = gpu_ctx.last_render_counts()
+        gpu_ctx.render(sc, cam, W, H, vfov=60.0, bounces=1)
+        nv1, nt1 = gpu_ctx.last_render_counts()
+    finally:
+        gpu_ctx.set_render_counting(False)
+    assert np.array_equal(img, counted)
+    rays = rt.rays_camera_tiles(cam, 60.0, W, H, 1, 0, W * H)
+    _, nv, tt = gpu_ctx.closest_hit_stats(sc, rt.rays_to_device(rays))
+    assert nv1 == int(nv.to(torch.int64).sum().item()) and nt1 == int(tt.to(torch.int64).sum().item())
+    assert nv4 > nv1 > 0 and nt4 >= nt1
+    with pytest.raises(rt.RtError):
+        gpu_ctx.render(sc, cam, W, H, vfov=60.0, bounces=1)
+        gpu_ctx.last_render_counts()                     # the last render was not counted
+    sc.release()

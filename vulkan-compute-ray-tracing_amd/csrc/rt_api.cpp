@@ -88,6 +88,7 @@ extern "C" int rt_destroy(rt_ctx* ctx) {
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_cursors) (void)hipFree(ctx->d_cursors);
+    if (ctx->d_render_sums) (void)hipFree(ctx->d_render_sums);
     for (hipEvent_t e : ctx->render_ev)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->query_ev)
@@ -211,6 +212,7 @@ int render_common(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo, const r
         return set_error(ctx, RT_ERR_INVALID, "render: vfov must lie in (0, 180) degrees");
     RT_HIP(ctx, hipSetDevice(ctx->device));
     ctx->last_timed = false;
+    ctx->last_counted = false;
     int rc = render_impl(ctx, scene, *ubo, *params, frames, (uint32_t*)target, (const uint32_t*)accum_in,
                          (uint32_t*)accum_out, w, h, (hipStream_t)stream);
     if (rc) return rc;
@@ -218,6 +220,7 @@ int render_common(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo, const r
     ctx->last_secondary = UINT64_MAX;   // read lazily from the counters
     ctx->last_bounces = params->bounces;
     ctx->last_timed = ctx->time_render && params->bounces > 0;
+    ctx->last_counted = ctx->count_render && ctx->d_render_sums;
     return RT_OK;
 }
 
@@ -237,6 +240,27 @@ extern "C" int rt_render_frames(rt_ctx* ctx, const rt_scene* scene, const rt_ubo
 extern "C" int rt_render_set_timing(rt_ctx* ctx, int enable) {
     if (!ctx) return RT_ERR_INVALID;
     ctx->time_render = enable != 0;
+    return RT_OK;
+}
+
+extern "C" int rt_render_set_counting(rt_ctx* ctx, int enable) {
+    if (!ctx) return RT_ERR_INVALID;
+    if (enable && !ctx->d_render_sums) {
+        RT_HIP(ctx, hipSetDevice(ctx->device));
+        RT_HIP(ctx, hipMalloc((void**)&ctx->d_render_sums, 2 * sizeof(unsigned long long)));
+    }
+    ctx->count_render = enable != 0;
+    return RT_OK;
+}
+
+extern "C" int rt_render_last_counts(rt_ctx* ctx, uint64_t* node_records, uint64_t* tri_records) {
+    if (!ctx || !node_records || !tri_records) return RT_ERR_INVALID;
+    if (!ctx->last_counted) return set_error(ctx, RT_ERR_INVALID, "rt_render_last_counts: the last render was not counted");
+    unsigned long long h[2] = {0, 0};
+    RT_HIP(ctx, hipSetDevice(ctx->device));
+    RT_HIP(ctx, hipMemcpy(h, ctx->d_render_sums, sizeof h, hipMemcpyDeviceToHost));
+    *node_records = h[0];
+    *tri_records = h[1];
     return RT_OK;
 }
 

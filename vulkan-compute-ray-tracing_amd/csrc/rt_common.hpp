@@ -242,6 +242,11 @@ struct rt_ctx {
     hipEvent_t query_ev[4] = {};
     bool last_query_timed = false;
     bool last_query_sorted = false;
+    // opt-in record counting of renders: their traversals run the counting
+    // variant and add their node and triangle record totals here
+    bool count_render = false;
+    bool last_counted = false;
+    unsigned long long* d_render_sums = nullptr;
 };
 
 namespace rtamd {
@@ -283,6 +288,7 @@ struct TraceArgs {
     bool render = false;        // a bounce of the render's wavefront loop
     const uint32_t* order = nullptr;   // packed traversal: trace order, or index order
     uint32_t* cursor = nullptr;        // kCursorBlock words for the ray cursors, or null: ctx's ring
+    unsigned long long* sums = nullptr;   // counting variant: record totals (2 words), or null
     bool no_alloc = false;             // a variant that must not allocate: refuse what would
 };
 int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);

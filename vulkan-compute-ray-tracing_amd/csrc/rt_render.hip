@@ -304,6 +304,11 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
     if (bounces + 1 > 64) return set_error(ctx, RT_ERR_INVALID, "render: at most 63 bounces");
     rc = launch_fill_u32(ctx, counters, 0u, 64u, s);
     if (rc) return rc;
+    unsigned long long* sums = ctx->count_render ? ctx->d_render_sums : nullptr;
+    if (sums) {
+        rc = launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(sums), 0u, 4u, s);
+        if (rc) return rc;
+    }
 
     // timing mode: ev[0] before generate, ev[1+2b] / ev[2+2b] around bounce
     // b's traversal, ev[1+2*bounces] after store
@@ -323,6 +328,7 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
     for (uint32_t b = 0; b < bounces; ++b) {
         const uint32_t* n_dev = b == 0 ? nullptr : counters + b;
         TraceArgs ta{sc, rays[b & 1], P, n_dev, hits, prm.spheres, nullptr, nullptr, true};
+        if (use_packed) ta.sums = sums;
         if (ev) RT_HIP(ctx, hipEventRecord(ev[1 + 2 * b], s));
         // secondary bounces start on surfaces all over the scene: trace them
         // in origin order (camera rays share one origin and are coherent)

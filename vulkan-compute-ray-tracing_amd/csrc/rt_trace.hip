@@ -182,8 +182,9 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 template <int STACK, int LAYOUT, int RING>
 int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
     const rt_scene* sc = a.scene;
-    auto kern = a.render ? k_trace_render<kBlock, STACK, LAYOUT, RING>
-                : stats    ? k_trace_packed<kBlock, STACK, LAYOUT, true, RING>
+    // the counting variant also serves a render whose record totals are asked for
+    auto kern = stats      ? k_trace_packed<kBlock, STACK, LAYOUT, true, RING>
+                : a.render ? k_trace_render<kBlock, STACK, LAYOUT, RING>
                            : k_trace_packed<kBlock, STACK, LAYOUT, false, RING>;
     int per_cu = 0;
     RT_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
@@ -223,7 +224,7 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
     TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->frame,
                   LAYOUT == kLayoutW4 || LAYOUT == kLayoutW4Joined ? sc->root4 : sc->root_ref, sc->tri_off};
     TraceIo io{a.rays, a.n_dev, a.hits, a.node_visits, a.tri_tests, cursor, a.n, refill, a.order, segs,
-               drain_steal(ctx)};
+               drain_steal(ctx), a.sums};
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, ts, io, spill);
     RT_HIP(ctx, hipGetLastError());
     return RT_OK;
@@ -339,7 +340,7 @@ int resolved_layout(const rt_ctx* ctx) {
 int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s) {
     const rt_scene* sc = a.scene;
     if (!sc->fast) return set_error(ctx, RT_ERR_UNSUPPORTED, "packed traversal needs a tree that encloses its triangles");
-    const bool stats = a.node_visits && a.tri_tests;
+    const bool stats = (a.node_visits && a.tri_tests) || a.sums;
     if (resolved_layout(ctx) == RT_LAYOUT_W4) {
         // up to three deferred children per 4-wide node on the path
         const uint32_t need = 3u * sc->height4 + 1u;

@@ -201,6 +201,7 @@ struct TraceIo {
     const uint32_t* order;    // trace order (ray indices), or null: index order
     uint32_t segs;            // segments of the ray range (1 or kSegments), see below
     uint32_t steal;           // idle lanes that make a drained wave share work (65: never)
+    unsigned long long* sums; // counting variant: totals of node and triangle records, or null
 };
 
 __device__ __forceinline__ float lo16(uint32_t w) { return (float)(w & 0xffffu); }
@@ -320,6 +321,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     float best_t = 0.0f, bu = 0.0f, bv = 0.0f;
     int32_t best = -1;
     uint32_t nv = 0, nt = 0;
+    uint64_t sum_nv = 0, sum_nt = 0;   // STATS: this lane's totals over its rays
     int sp = 0;
     int32_t cur = 0;
     // work sharing (STEAL): the stack holds entries [bot, sp); the lane's
@@ -519,7 +521,11 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                 const int32_t id = best >= 0 ? l_triid[best] : -1;
                 if (id >= 0) store_hit_stream(l_hits, ray_i, id, best_t, bu, bv);
                 else store_hit_stream(l_hits, ray_i, -1, 0.0f, 0.0f, 0.0f);
-                if (STATS) { io.node_visits[ray_i] = nv; io.tri_tests[ray_i] = nt; }
+                if (STATS) {
+                    if (io.node_visits) { io.node_visits[ray_i] = nv; io.tri_tests[ray_i] = nt; }
+                    sum_nv += nv;
+                    sum_nt += nt;
+                }
                 busy = false;
             }
         }
@@ -727,6 +733,19 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     busy = true;
                     sp = -1;
                 }
+            }
+        }
+    }
+    if constexpr (STATS) {
+        if (io.sums) {
+            // the wave's totals, then one atomic pair per wave
+            for (int off = 32; off > 0; off >>= 1) {
+                sum_nv += (uint64_t)__shfl_xor((unsigned long long)sum_nv, off);
+                sum_nt += (uint64_t)__shfl_xor((unsigned long long)sum_nt, off);
+            }
+            if (lane == 0) {
+                atomicAdd(io.sums, (unsigned long long)sum_nv);
+                atomicAdd(io.sums + 1, (unsigned long long)sum_nt);
             }
         }
     }

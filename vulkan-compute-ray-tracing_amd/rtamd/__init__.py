@@ -188,7 +188,8 @@ EXPORTS = [
     "rt_scene_check", "rt_tuning_set", "rt_tuning_get",
     "rt_scene_release", "rt_closest_hit", "rt_closest_hit_stats", "rt_closest_hit_noalloc", "rt_render",
     "rt_render_frames",
-    "rt_render_last_rays", "rt_render_set_timing", "rt_render_last_stats", "rt_query_last_stats", "rt_bvh_build", "rt_bvh_build_seeded", "rt_bvh_build_device", "rt_scene_lights",
+    "rt_render_last_rays", "rt_render_set_timing", "rt_render_last_stats", "rt_render_set_counting",
+    "rt_render_last_counts", "rt_query_last_stats", "rt_bvh_build", "rt_bvh_build_seeded", "rt_bvh_build_device", "rt_scene_lights",
     "rt_frame_loop_create", "rt_frame_loop_destroy", "rt_frame_loop_step", "rt_frame_loop_display",
     "rt_frame_loop_images", "rt_frame_loop_read", "rt_postprocess_gamma", "rt_gamma_table", "rt_image_write",
     "rt_batch_create", "rt_batch_set_config", "rt_batch_get_config", "rt_batch_launch", "rt_batch_autotune",
@@ -238,6 +239,8 @@ def lib():
         "rt_render_last_rays": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "rt_render_set_timing": (i32, [P, i32]),
         "rt_render_last_stats": (i32, [P, ctypes.POINTER(RenderStats)]),
+        "rt_render_set_counting": (i32, [P, i32]),
+        "rt_render_last_counts": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "rt_query_last_stats": (i32, [P, ctypes.POINTER(QueryStats)]),
         "rt_batch_create": (i32, [P, ctypes.POINTER(BatchItem), u32, ctypes.POINTER(BatchConfig), ctypes.POINTER(P)]),
         "rt_batch_set_config": (i32, [P, P, ctypes.POINTER(BatchConfig)]),
@@ -610,6 +613,15 @@ class Context:
         st = RenderStats()
         self.check(lib().rt_render_last_stats(self._h, ctypes.byref(st)), "rt_render_last_stats")
         return st
+
+    def set_render_counting(self, enable: bool):
+        self.check(lib().rt_render_set_counting(self._h, 1 if enable else 0), "rt_render_set_counting")
+
+    def last_render_counts(self) -> tuple[int, int]:
+        """(node records, triangle records) the last counted render's rays read."""
+        nv, nt = ctypes.c_uint64(), ctypes.c_uint64()
+        self.check(lib().rt_render_last_counts(self._h, ctypes.byref(nv), ctypes.byref(nt)), "rt_render_last_counts")
+        return int(nv.value), int(nt.value)
 
 
 class Batch:
