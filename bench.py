@@ -160,7 +160,8 @@ def run_counter_passes(args) -> dict | None:
     agg = {}
     for k, counters in enumerate(PMC_PASSES):
         d = tempfile.mkdtemp(prefix="rtamd_pmc_")
-        cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc", *counters, "-f", "csv", "-d", d, "-o", "pmc",
+        # a pass takes a few seconds; a run that exceeds the limit is killed and its pass reads as absent
+        cmd = ["timeout", "-s", "KILL", "120", "rocprofv3", "--pmc", *counters, "-f", "csv", "-d", d, "-o", "pmc",
                "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--tris", str(args.tris),
                "--rays", str(args.rays), "--layout", args.layout, "--refill", str(args.refill)]
         if args.no_render:
