@@ -24,6 +24,13 @@
  *   lights[] read past the end returns zeros (robust buffer access)
  * These are the points that only a run of the shader on a Vulkan driver
  * could pin ("parity unpinned" for the driver's own choices).
+ *
+ * Every function restated here is pinned apart from this file's own code by
+ * tests/test_oracle_kat.py, test_oracle_kat_paths.py, test_oracle_traversal.py
+ * and test_oracle_lbvh.py: the RNG's known-answer table, closed forms of the
+ * box, triangle and sphere tests, the camera, scatter, light sampling,
+ * cosine direction and colour rule, brute-force traversals, and mutants of
+ * this file with one planted mistake that those checks must reject.
  */
 #include <math.h>
 #include <stdint.h>
