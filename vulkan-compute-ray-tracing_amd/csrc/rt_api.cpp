@@ -221,6 +221,7 @@ int render_common(rt_ctx* ctx, const rt_scene* scene, const rt_ubo* ubo, const r
     ctx->last_bounces = params->bounces;
     ctx->last_timed = ctx->time_render && params->bounces > 0;
     ctx->last_counted = ctx->count_render && ctx->d_render_sums;
+    ctx->last_render_stream = (hipStream_t)stream;
     return RT_OK;
 }
 
@@ -258,7 +259,9 @@ extern "C" int rt_render_last_counts(rt_ctx* ctx, uint64_t* node_records, uint64
     if (!ctx->last_counted) return set_error(ctx, RT_ERR_INVALID, "rt_render_last_counts: the last render was not counted");
     unsigned long long h[2] = {0, 0};
     RT_HIP(ctx, hipSetDevice(ctx->device));
-    RT_HIP(ctx, hipMemcpy(h, ctx->d_render_sums, sizeof h, hipMemcpyDeviceToHost));
+    // the totals are final once the render's stream has run it
+    RT_HIP(ctx, hipMemcpyAsync(h, ctx->d_render_sums, sizeof h, hipMemcpyDeviceToHost, ctx->last_render_stream));
+    RT_HIP(ctx, hipStreamSynchronize(ctx->last_render_stream));
     *node_records = h[0];
     *tri_records = h[1];
     return RT_OK;

@@ -247,6 +247,7 @@ struct rt_ctx {
     bool count_render = false;
     bool last_counted = false;
     unsigned long long* d_render_sums = nullptr;
+    hipStream_t last_render_stream = nullptr;   // the stream the counted render ran on
 };
 
 namespace rtamd {
