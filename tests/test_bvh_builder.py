@@ -1,6 +1,7 @@
 """The library's BVH builder (rt_bvh_build*) against the Python restatement of
 Bvh::createBvh (oracle/bvh_ref.py), plus the structural invariants the
 reference algorithm fixes (Bvh.h:141-209)."""
+import os
 import subprocess
 import sys
 
@@ -8,6 +9,8 @@ import numpy as np
 import pytest
 
 from oracle import bvh_ref
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def grid_mesh(nx, ny, mat=0):
@@ -81,7 +84,7 @@ def test_process_rand_builder_is_seed_one_in_fresh_process():
         "b=rtamd.bvh_build(s.triangles, seed=1);"
         "print(int(all(np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8),"
         " np.ascontiguousarray(b[k]).view(np.uint8)) for k in a.dtype.names)))"
-    ) % (sys.path[0], sys.path[1])
+    ) % (ROOT, os.path.join(ROOT, "vulkan-compute-ray-tracing_amd"))
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True).stdout
     assert out.strip() == "1"
 
