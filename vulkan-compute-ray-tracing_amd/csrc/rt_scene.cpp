@@ -234,8 +234,8 @@ void widen_nodes(PackedTree& out) {
             if (k < n) {
                 // the binary record's words hold (lo x, lo y), (lo z, hi x),
                 // (hi y, hi z); the 4-wide one pairs each axis' planes,
-                // (lo x, hi x), (lo y, hi y), (lo z, hi z), so one rotate by
-                // 16 bits puts the plane a ray enters first in the low half
+                // (lo x, hi x), (lo y, hi y), (lo z, hi z), so a ray picks the
+                // plane it enters by and the one it leaves by from one word
                 const uint32_t lx = items[k].w0 & 0xffffu, ly = items[k].w0 >> 16;
                 const uint32_t lz = items[k].w1 & 0xffffu, hx = items[k].w1 >> 16;
                 const uint32_t hy = items[k].w2 & 0xffffu, hz = items[k].w2 >> 16;

@@ -70,13 +70,15 @@ __device__ __forceinline__ bool slab(float lox, float loy, float loz, float hix,
 // Slab test of a 16-bit box in the frame's units with each axis' planes in
 // the order the ray meets them: (nx, ny, nz) the planes it enters by,
 // (fx, fy, fz) those it leaves by (for inv >= 0 the low plane first, for
-// inv < 0 the high one), so no min/max pairs are needed.  A plane q is met at
-// t = q * inv + c with c = -bo * inv, one fused multiply-add.  Against the
-// exact (q - bo) * inv, the rounding of c moves a plane by |bo| * 2^-24
-// quanta and that of bo by |bo| * 2^-23; frame_far keeps |bo| < 2^21, so
-// the planes move by under 0.4 quanta, inside the one-quantum padding every
-// quantized bound carries.  The remaining errors are relative (inv, the
-// final rounding) and slab_accept covers them.
+// inv < 0 the high one), so no min/max pairs are needed.  Planes arrive
+// biased (biased_plane): the float 2^23 + q, met at t = (2^23 + q) * inv + c
+// with c = -(2^23 + bo) * inv (plane_bias), one fused multiply-add.  The fma
+// is exact before its final rounding, so against the exact (q - bo) * inv
+// the only absolute error is the rounding of c, (2^23 + |bo|) * 2^-24
+// quanta; frame_far keeps |bo| < 2^21, so the planes move by under 0.63
+// quanta, inside the one-quantum padding every quantized bound carries.  The
+// remaining errors are relative (inv, the final rounding) and slab_accept
+// covers them.
 __device__ __forceinline__ bool slab_ordered(float nx, float ny, float nz, float fx, float fy, float fz,
                                              const f3& c, const f3& inv, float tmin, float tmax, float& tnear) {
     const float tnx = __builtin_fmaf(nx, inv.x, c.x), tfx = __builtin_fmaf(fx, inv.x, c.x);
@@ -86,6 +88,22 @@ __device__ __forceinline__ bool slab_ordered(float nx, float ny, float nz, float
     const float farv = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
     tnear = nearv;
     return slab_accept(nearv, farv);
+}
+
+// A 16-bit plane of a 4-wide record word as the float 2^23 + q: one byte
+// permute puts the half the selector names under the exponent byte 0x4B
+// (2^23, whose unit in the last place is 1), so the float holds q exactly
+// with no conversion.  kSelLow takes the word's low half, kSelHigh its high half.
+constexpr uint32_t kPlaneExponent = 0x4B000000u;
+constexpr uint32_t kSelLow = 0x070C0100u;
+constexpr uint32_t kSelHigh = 0x070C0302u;
+__device__ __forceinline__ float biased_plane(uint32_t w, uint32_t sel) {
+    return __uint_as_float(__builtin_amdgcn_perm(kPlaneExponent, w, sel));
+}
+
+// c of a biased plane (slab_ordered): -(2^23 + bo) * inv, rounded once.
+__device__ __forceinline__ float plane_bias(double bo, float inv) {
+    return (float)(-(0x1p23 + bo) * (double)inv);
 }
 
 // triIntersect + hit_triangle's accept test on a packed triangle.
@@ -152,7 +170,8 @@ constexpr float kFarOrigin = 2097152.0f;
 // frame's box, and t_shift records how far.  Returns false when the ray
 // misses that box within [tmin, tmax]: it can hit no node box (a tree whose
 // root is a single triangle has no node box, and the frame bounds nothing).
-__device__ __forceinline__ bool frame_far(const RayIn& r, const QFrame& q, f3& bo, float& t_shift) {
+// bod receives the advanced origin in double, for plane_bias.
+__device__ __forceinline__ bool frame_far(const RayIn& r, const QFrame& q, f3& bo, double* bod, float& t_shift) {
     const double o[3] = {((double)r.o.x - (double)q.ox) / (double)q.sx, ((double)r.o.y - (double)q.oy) / (double)q.sy,
                          ((double)r.o.z - (double)q.oz) / (double)q.sz};
     const double d[3] = {(double)r.d.x / (double)q.sx, (double)r.d.y / (double)q.sy, (double)r.d.z / (double)q.sz};
@@ -170,7 +189,8 @@ __device__ __forceinline__ bool frame_far(const RayIn& r, const QFrame& q, f3& b
     if (!(t0 <= t1)) return false;
     t_shift = (float)t0;
     const double ts = (double)t_shift;
-    bo = f3{(float)(o[0] + ts * d[0]), (float)(o[1] + ts * d[1]), (float)(o[2] + ts * d[2])};
+    for (int k = 0; k < 3; ++k) bod[k] = o[k] + ts * d[k];
+    bo = f3{(float)bod[0], (float)bod[1], (float)bod[2]};
     return true;
 }
 
@@ -311,10 +331,20 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
     RayIn r{};
     f3 bo{0.0f, 0.0f, 0.0f};        // ray origin in the box frame
     f3 inv{0.0f, 0.0f, 0.0f};       // reciprocal direction in the box frame
-    // 4-wide records: 16 where the ray runs toward -axis (its entry plane is
-    // then the word's high half), so a rotate puts the entry plane low
-    uint32_t rot_x = 0, rot_y = 0, rot_z = 0;
-    f3 cq{0.0f, 0.0f, 0.0f};        // -bo * inv: 4-wide box planes are q * inv + cq
+    // 4-wide records: the byte-permute selectors of each axis' entry and exit
+    // planes (the low half first where the ray runs toward +axis)
+    uint32_t sn_x = kSelLow, sn_y = kSelLow, sn_z = kSelLow;
+    uint32_t sf_x = kSelHigh, sf_y = kSelHigh, sf_z = kSelHigh;
+    auto set_planes = [&](float ix, float iy, float iz) {
+        const bool nx = __float_as_uint(ix) >> 31, ny = __float_as_uint(iy) >> 31, nz = __float_as_uint(iz) >> 31;
+        sn_x = nx ? kSelHigh : kSelLow;
+        sf_x = nx ? kSelLow : kSelHigh;
+        sn_y = ny ? kSelHigh : kSelLow;
+        sf_y = ny ? kSelLow : kSelHigh;
+        sn_z = nz ? kSelHigh : kSelLow;
+        sf_z = nz ? kSelLow : kSelHigh;
+    };
+    f3 cq{0.0f, 0.0f, 0.0f};        // plane_bias: 4-wide box planes are biased_plane * inv + cq
     // node tests run on t - t_shift (frame_far); tmin_rel and best_rel are
     // tmin and the closest hit's t on that scale (equal to them when t_shift = 0)
     float t_shift = 0.0f, tmin_rel = 0.0f, best_rel = 0.0f;
@@ -413,13 +443,11 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                 float k0 = __builtin_inff(), k1 = __builtin_inff(), k2 = __builtin_inff(), k3 = __builtin_inff();
                 float tn;
                 int nh = 0;
-                // child k's words, each rotated so the entry plane is the low half
+                // child k's words: entry and exit planes by the ray's selectors
                 auto box = [&](uint32_t wx, uint32_t wy, uint32_t wz, float& key) {
-                    const uint32_t x = __builtin_amdgcn_alignbit(wx, wx, rot_x);
-                    const uint32_t y = __builtin_amdgcn_alignbit(wy, wy, rot_y);
-                    const uint32_t z = __builtin_amdgcn_alignbit(wz, wz, rot_z);
-                    if (slab_ordered(lo16(x), lo16(y), lo16(z), hi16(x), hi16(y), hi16(z), cq, inv, tmin_rel,
-                                     best_rel, tn)) {
+                    if (slab_ordered(biased_plane(wx, sn_x), biased_plane(wy, sn_y), biased_plane(wz, sn_z),
+                                     biased_plane(wx, sf_x), biased_plane(wy, sf_y), biased_plane(wz, sf_z), cq,
+                                     inv, tmin_rel, best_rel, tn)) {
                         key = tn;
                         nh++;
                     }
@@ -580,13 +608,18 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                     if (LAYOUT == kLayoutQ16 || kW4) {
                         bo = f3{(r.o.x - q.ox) * q.rx, (r.o.y - q.oy) * q.ry, (r.o.z - q.oz) * q.rz};
                         inv = f3{inv.x * q.sx, inv.y * q.sy, inv.z * q.sz};
+                        // the origin in the frame in double, as the quantizer placed the bounds
+                        double bod[3] = {0.0, 0.0, 0.0};
+                        if (kW4) {
+                            bod[0] = ((double)r.o.x - (double)q.ox) / (double)q.sx;
+                            bod[1] = ((double)r.o.y - (double)q.oy) / (double)q.sy;
+                            bod[2] = ((double)r.o.z - (double)q.oz) / (double)q.sz;
+                        }
                         if (!(fmaxf(fmaxf(fabsf(bo.x), fabsf(bo.y)), fabsf(bo.z)) < kFarOrigin))
-                            reach = frame_far(r, q, bo, t_shift);
+                            reach = frame_far(r, q, bo, bod, t_shift);
+                        if (kW4) cq = f3{plane_bias(bod[0], inv.x), plane_bias(bod[1], inv.y), plane_bias(bod[2], inv.z)};
                     }
-                    rot_x = (__float_as_uint(inv.x) >> 27) & 16u;
-                    rot_y = (__float_as_uint(inv.y) >> 27) & 16u;
-                    rot_z = (__float_as_uint(inv.z) >> 27) & 16u;
-                    cq = f3{-(bo.x * inv.x), -(bo.y * inv.y), -(bo.z * inv.z)};
+                    set_planes(inv.x, inv.y, inv.z);
                     tmin_rel = r.tmin - t_shift;
                     best_rel = r.tmax - t_shift;
                     best_t = r.tmax; bu = 0.0f; bv = 0.0f; best = -1;
@@ -664,9 +697,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene& sc, const Tra
                         r.tmin = tmn;
                         inv = f3{ix, iy, iz};
                         cq = f3{cx, cy, cz};
-                        rot_x = (__float_as_uint(ix) >> 27) & 16u;
-                        rot_y = (__float_as_uint(iy) >> 27) & 16u;
-                        rot_z = (__float_as_uint(iz) >> 27) & 16u;
+                        set_planes(ix, iy, iz);
                         t_shift = tsh;
                         tmin_rel = tminr;
                         best_rel = bestr;
