@@ -225,8 +225,9 @@ typedef struct rt_tuning {
     uint32_t node_layout;   /* RT_LAYOUT_*: binary 64-byte fp32 or 32-byte 16-bit
                                nodes, or 4-wide 64-byte 16-bit nodes            */
     uint32_t refill;        /* idle lanes (1..64) that make a wave fetch rays;
-                               0 = by scene: 24 from 2^17 triangles up (long
-                               traversals), 48 below (short ones)                */
+                               0 = by scene: from 2^17 triangles up (long
+                               traversals) 28 for queries and 24 for render
+                               bounces, 48 below (short ones)                    */
     uint32_t blocks_per_cu; /* persistent blocks per CU (0 = occupancy limit)     */
     uint32_t stack_lds;     /* 4-wide layout: 0 = whole traversal stack in LDS;
                                8, 16 or 32 = that many top entries in LDS, deeper

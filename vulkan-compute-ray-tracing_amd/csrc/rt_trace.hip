@@ -214,12 +214,14 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
     // idle).  The loop is VALU-bound, so idle lanes waste issue slots: a
     // large scene's long traversals want lanes refilled early, while a small
     // scene's short ones finish so fast that each refill's fetches are better
-    // shared by more rays.  Measured: 24 is best from 2^17 triangles up
-    // (4M-triangle query 6.35 ms against 6.72 at 16 and 6.46 at 32, with the
-    // drain's work sharing taking over the lanes idle at the end), 48 below
+    // shared by more rays.  Measured from 2^17 triangles up, with the drain's
+    // work sharing taking over the lanes idle at the end: a query is best at
+    // 28 (4M-triangle query 6.06-6.08 ms against 6.10-6.11 at 24, 6.12-6.14
+    // at 32, 6.27 at 20), a render's bounces at 24 (1080p render 32.7 ms
+    // against 32.8 at 28 and 33.4 at 32).  Below 2^17 triangles 48 is best
     // (-1..4% against 32).
     uint32_t refill = ctx->tuning.refill ? ctx->tuning.refill
-                                         : (sc->info.num_triangles >= (1u << 17) ? 24u : 48u);
+                                         : (sc->info.num_triangles >= (1u << 17) ? (a.render ? 24u : 28u) : 48u);
     if (refill > 64u) refill = 64u;
     TraceScene ts{sc->d_pnodes, sc->d_qnodes, sc->d_w4nodes, sc->d_ptris, sc->d_ptri_id, sc->frame,
                   LAYOUT == kLayoutW4 || LAYOUT == kLayoutW4Joined ? sc->root4 : sc->root_ref, sc->tri_off};
