@@ -69,8 +69,8 @@ extern "C" int rt_create(rt_ctx** out, int device) {
     c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     // the allocation-free query's scratch: its device address, once per device
     (void)noalloc_setup(device);
-    if (hipMalloc((void**)&c->d_counters, 64 * sizeof(uint32_t)) != hipSuccess) { delete c; return RT_ERR_NOMEM; }
-    if (hipMemset(c->d_counters, 0, 64 * sizeof(uint32_t)) != hipSuccess ||
+    if (hipMalloc((void**)&c->d_counters, kRenderCounters * sizeof(uint32_t)) != hipSuccess) { delete c; return RT_ERR_NOMEM; }
+    if (hipMemset(c->d_counters, 0, kRenderCounters * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void**)&c->d_cursors, kCursorSlots * sizeof(uint32_t)) != hipSuccess) {
         (void)hipFree(c->d_counters);
         delete c;

@@ -225,7 +225,7 @@ struct rt_ctx {
     static constexpr int kWorkspaces = 6;
     void* ws[kWorkspaces] = {};
     size_t ws_bytes[kWorkspaces] = {};
-    uint32_t* d_counters = nullptr;   // small zero-filled counter block
+    uint32_t* d_counters = nullptr;   // kRenderCounters words: a render's live rays per bounce
     uint32_t* d_cursors = nullptr;    // ring of persistent-kernel ray cursors
     uint32_t cursor_next = 0;
     rt_tuning tuning{};               // 0 fields = defaults
@@ -267,6 +267,8 @@ static_assert(kSegments == 1u << kSegmentShift, "segments are a power of two");
 constexpr uint32_t kCursorStride = 32;
 constexpr uint32_t kCursorBlock = kSegments * kCursorStride;
 constexpr uint32_t kCursorBlocks = 256;
+// A render's per-bounce counters (rt_ctx::d_counters): [b] = live rays entering bounce b.
+constexpr uint32_t kRenderCounters = 64;
 constexpr uint32_t kCursorSlots = kCursorBlocks * kCursorBlock;
 int ensure_workspace(rt_ctx* ctx, int slot, size_t bytes);
 
@@ -289,10 +291,14 @@ struct TraceArgs {
     bool render = false;        // a bounce of the render's wavefront loop
     const uint32_t* order = nullptr;   // packed traversal: trace order, or index order
     uint32_t* cursor = nullptr;        // kCursorBlock words for the ray cursors, or null: ctx's ring
+    bool cursor_zeroed = false;        // an earlier kernel on the stream zeroed cursor's segment words
     unsigned long long* sums = nullptr;   // counting variant: record totals (2 words), or null
     bool no_alloc = false;             // a variant that must not allocate: refuse what would
 };
 int launch_trace_packed(rt_ctx* ctx, const TraceArgs& a, hipStream_t s);
+// The next block of the context's ray-cursor ring (kCursorBlock words; the
+// traversal reads word k * kCursorStride of segment k).
+uint32_t* next_cursor(rt_ctx* ctx);
 // The trace order of a packed query (rt_trace.hip): sorts the rays by
 // origin cell into ctx's order workspace when ctx->tuning.ray_order asks
 // for it (or, by default, for large batches on large scenes); *order stays

@@ -14,7 +14,10 @@
 //   k_store              progressive 8-bit average per pixel (:375-379)
 //
 // Each path's arithmetic and RNG stream depend only on (pixel, sample), so
-// the queue order chosen by compaction never changes a result.
+// the queue order chosen by compaction never changes a result.  The counters
+// and ray cursors a stage needs at zero are cleared by the stage before it
+// on the same stream (k_generate for the first bounce, k_shade for the next),
+// so a frame launches no fill kernels.
 #include <cmath>
 
 #include "rt_common.hpp"
@@ -118,7 +121,17 @@ __device__ __forceinline__ void wave_append(bool keep, uint32_t* counter, rt_ray
 
 __global__ __launch_bounds__(256) void k_generate(uint32_t w, uint32_t h, uint32_t frames, uint32_t sample0, Camera cam,
                                                   rt_ray* __restrict__ rays, uint32_t* __restrict__ pid,
-                                                  float4* __restrict__ color, uint32_t* __restrict__ rng) {
+                                                  float4* __restrict__ color, uint32_t* __restrict__ rng,
+                                                  uint32_t* __restrict__ counters, uint32_t* __restrict__ cursor,
+                                                  unsigned long long* __restrict__ sums) {
+    // the render's bounce counters, record totals and first ray cursors
+    // start at zero (written here instead of by fill launches)
+    if (blockIdx.x == 0) {
+        const uint32_t t = threadIdx.x;
+        if (t < kRenderCounters) counters[t] = 0u;
+        if (cursor && t < kSegments) cursor[t * kCursorStride] = 0u;
+        if (sums && t < 2) sums[t] = 0ull;
+    }
     const uint32_t pix_count = w * h;
     const uint32_t total = pix_count * frames;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
@@ -142,7 +155,10 @@ __global__ __launch_bounds__(256) void k_shade(
     uint32_t n_host, const rt_hit* __restrict__ hits, rt_ray* __restrict__ out_rays, uint32_t* __restrict__ out_pid,
     uint32_t* __restrict__ n_out, float4* __restrict__ color, uint32_t* __restrict__ rngs,
     const rt_triangle* __restrict__ tris, const rt_material* __restrict__ mats, const rt_light* __restrict__ lights,
-    uint32_t nlights, const rt_sphere* __restrict__ spheres, uint32_t ntri, uint32_t last) {
+    uint32_t nlights, const rt_sphere* __restrict__ spheres, uint32_t ntri, uint32_t last,
+    uint32_t* __restrict__ next_cursor) {
+    // the next bounce's traversal finds its ray cursors zeroed (no fill launch)
+    if (next_cursor && blockIdx.x == 0 && threadIdx.x < kSegments) next_cursor[threadIdx.x * kCursorStride] = 0u;
     const uint32_t n = n_in ? *n_in : n_host;
     const uint32_t stride = gridDim.x * blockDim.x;
     // every lane of a wave runs the same number of iterations so the
@@ -301,14 +317,11 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
     uint32_t* rng = (uint32_t*)base; base += (size_t)P * 4;
     uint32_t* counters = ctx->d_counters;   // [b] = live rays entering bounce b
 
-    if (bounces + 1 > 64) return set_error(ctx, RT_ERR_INVALID, "render: at most 63 bounces");
-    rc = launch_fill_u32(ctx, counters, 0u, 64u, s);
-    if (rc) return rc;
+    if (bounces + 1 > kRenderCounters) return set_error(ctx, RT_ERR_INVALID, "render: at most 63 bounces");
     unsigned long long* sums = ctx->count_render ? ctx->d_render_sums : nullptr;
-    if (sums) {
-        rc = launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(sums), 0u, 4u, s);
-        if (rc) return rc;
-    }
+    // each bounce's packed traversal takes a cursor block that the kernel
+    // before it (generate, then the previous bounce's shading) zeroed
+    uint32_t* cursor = use_packed ? next_cursor(ctx) : nullptr;
 
     // timing mode: ev[0] before generate, ev[1+2b] / ev[2+2b] around bounce
     // b's traversal, ev[1+2*bounces] after store
@@ -322,13 +335,17 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
     Camera cam = make_camera(ubo, prm.vfov_deg, w, h);
     const uint32_t cu_cap = (uint32_t)ctx->num_cus * 16u;
     hipLaunchKernelGGL(k_generate, dim3(grid_for(P, cu_cap)), dim3(256), 0, s, w, h, frames, ubo.currentSample, cam,
-                       rays[0], pids[0], color, rng);
+                       rays[0], pids[0], color, rng, counters, cursor, sums);
     RT_HIP(ctx, hipGetLastError());
 
     for (uint32_t b = 0; b < bounces; ++b) {
         const uint32_t* n_dev = b == 0 ? nullptr : counters + b;
         TraceArgs ta{sc, rays[b & 1], P, n_dev, hits, prm.spheres, nullptr, nullptr, true};
-        if (use_packed) ta.sums = sums;
+        if (use_packed) {
+            ta.sums = sums;
+            ta.cursor = cursor;
+            ta.cursor_zeroed = true;
+        }
         if (ev) RT_HIP(ctx, hipEventRecord(ev[1 + 2 * b], s));
         // secondary bounces start on surfaces all over the scene: trace them
         // in origin order (camera rays share one origin and are coherent)
@@ -341,10 +358,11 @@ int render_impl(rt_ctx* ctx, const rt_scene* sc, const rt_ubo& ubo, const rt_ren
         rc = launch_sphere_pass(ctx, ta, s);
         if (rc) return rc;
         if (ev) RT_HIP(ctx, hipEventRecord(ev[2 + 2 * b], s));
+        cursor = use_packed && b + 1 < bounces ? next_cursor(ctx) : nullptr;
         hipLaunchKernelGGL(k_shade, dim3(grid_for(P, cu_cap)), dim3(256), 0, s, rays[b & 1], pids[b & 1], n_dev, P,
                            hits, rays[(b + 1) & 1], pids[(b + 1) & 1], counters + b + 1, color, rng, sc->d_tris,
                            sc->d_mats, sc->d_lights, sc->info.num_lights, sc->d_spheres, sc->info.num_triangles,
-                           (uint32_t)(b + 1 == bounces));
+                           (uint32_t)(b + 1 == bounces), cursor);
         RT_HIP(ctx, hipGetLastError());
     }
     hipLaunchKernelGGL(k_store, dim3(grid_for((uint64_t)w * h, cu_cap)), dim3(256), 0, s, w * h, frames,

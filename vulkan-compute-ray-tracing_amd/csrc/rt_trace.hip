@@ -114,14 +114,6 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t value, uint32_t count) {
 
 constexpr int kBlock = 256;
 
-// Ray cursors for persistent launches: a ring of counters so launches on
-// different streams never share one; each is zeroed by a fill kernel on the
-// launch's own stream right before use.
-uint32_t* next_cursor(rt_ctx* ctx) {
-    uint32_t k = ctx->cursor_next++ % kCursorBlocks;
-    return ctx->d_cursors + (size_t)k * kCursorBlock;
-}
-
 // Trace order of a query: rays sorted by the Morton code of their origin's
 // cell in a 256^3 grid over the scene's node bounds (the 16-bit frame), so
 // the lanes of a wave start near each other and fetch the same upper nodes.
@@ -179,6 +171,19 @@ __global__ __launch_bounds__(256) void k_queue_keys(const rt_ray* __restrict__ r
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+}  // namespace
+
+// Ray cursors for persistent launches: a ring of counters so launches on
+// different streams never share one.  A launch zeroes its block with a fill
+// kernel on its own stream right before use, unless the caller had an
+// earlier kernel on that stream do it (TraceArgs::cursor_zeroed).
+uint32_t* next_cursor(rt_ctx* ctx) {
+    uint32_t k = ctx->cursor_next++ % kCursorBlocks;
+    return ctx->d_cursors + (size_t)k * kCursorBlock;
+}
+
+namespace {
+
 template <int STACK, int LAYOUT, int RING>
 int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) {
     const rt_scene* sc = a.scene;
@@ -206,8 +211,10 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
         spill = (int32_t*)ctx->ws[kWsSpill];
     }
     uint32_t* cursor = a.cursor ? a.cursor : next_cursor(ctx);
-    int rc = launch_fill_u32(ctx, cursor, 0u, kCursorBlock, s);
-    if (rc) return rc;
+    if (!(a.cursor && a.cursor_zeroed)) {
+        const int rc = launch_fill_u32(ctx, cursor, 0u, kCursorBlock, s);
+        if (rc) return rc;
+    }
     // segment the ray range per XCD when there is enough of it
     const uint32_t segs = (a.n_dev || a.n >= (1u << 16)) ? kSegments : 1u;
     // 1..64 idle lanes trigger a refetch (64: only when the whole wave is
