@@ -626,9 +626,16 @@ def main():
                     help="skip the counter passes (rocprofv3 --pmc runs of the query and render) and report the "
                          "committed profile's traffic instead")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--frame-loop-only", action="store_true",
+                    help="only the reference's own frame (the doge scene's frame loop, 60 frames); prints its "
+                         "figures as one JSON line, for kernel traces of that workload")
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args, host_threads())
+        return
+    if args.frame_loop_only:
+        import rtamd
+        print(json.dumps(bench_doge_frame_loop(rtamd.Context(0), device_tree=False)), flush=True)
         return
     if args.plain_only:
         args.no_render = args.no_batched = args.no_served = True
