@@ -324,6 +324,44 @@ def test_trace_order_is_invisible(rt, gpu_ctx):
         assert np.array_equal(out[mode][2], out[rt.ORDER_GIVEN][2])
 
 
+@pytest.mark.parametrize("origins", ["one", "few", "corners"])
+def test_sorted_order_traces_every_ray_once(rt, gpu_ctx, origins):
+    """The sorted trace order on degenerate origin sets: every ray from one
+    origin (one key for all), from 5 origins (long runs of equal keys), and
+    from the frame's corners and beyond (clamped cells).  Every ray is traced
+    exactly once: hits equal index order's byte for byte over a hit buffer
+    pre-filled with a sentinel."""
+    hs = rt.scene_clusters(1 << 18, 24)
+    scene = gpu_ctx.upload(hs)
+    n = 200003                                     # not a multiple of the wave
+    r = rt.rays_box(n, 25, 4.0)
+    rng = np.random.default_rng(26)
+    if origins == "one":
+        r["o"] = np.array([0.3, -0.2, 0.1], np.float32)
+    elif origins == "few":
+        pts = rng.uniform(-3.0, 3.0, (5, 3)).astype(np.float32)
+        r["o"] = pts[rng.integers(0, 5, n)]
+    else:
+        r["o"] = rng.choice([-9.0, 9.0], (n, 3)).astype(np.float32)
+        d = -r["o"] / np.linalg.norm(r["o"], axis=1, keepdims=True)
+        r["d"] = (d + rng.normal(scale=0.05, size=(n, 3))).astype(np.float32)
+    rays = rt.rays_to_device(r)
+    print(f"case: sorted trace order, origins {origins}, {n} rays", flush=True)
+    out = {}
+    for mode in (rt.ORDER_GIVEN, rt.ORDER_SORTED):
+        gpu_ctx.set_tuning(ray_order=mode)
+        try:
+            hits = torch.full((n, 4), -7.0, dtype=torch.float32, device="cuda")
+            gpu_ctx.closest_hit(scene, rays, hits)
+            torch.cuda.synchronize()
+            out[mode] = hits.cpu().numpy()
+        finally:
+            gpu_ctx.set_tuning()
+    assert not (out[rt.ORDER_SORTED][:, 1] == -7.0).any()
+    assert np.array_equal(out[rt.ORDER_SORTED].view(np.uint8), out[rt.ORDER_GIVEN].view(np.uint8))
+    assert (out[rt.ORDER_GIVEN][:, 0].view(np.int32) >= 0).mean() > 0.05
+
+
 def test_query_stage_timing(rt, gpu_ctx):
     hs = rt.scene_clusters(1 << 18, 23)
     scene = gpu_ctx.upload(hs)
