@@ -218,10 +218,12 @@ int launch_packed_t(rt_ctx* ctx, const TraceArgs& a, hipStream_t s, bool stats) 
     // segment the ray range per XCD when there is enough of it
     const uint32_t segs = (a.n_dev || a.n >= (1u << 16)) ? kSegments : 1u;
     // 1..64 idle lanes trigger a refetch (64: only when the whole wave is
-    // idle).  The loop is VALU-bound, so idle lanes waste issue slots: a
-    // large scene's long traversals want lanes refilled early, while a small
-    // scene's short ones finish so fast that each refill's fetches are better
-    // shared by more rays.  Measured from 2^17 triangles up, with the drain's
+    // idle).  The loop is bound by VALU issue, with each step's dependent
+    // record fetch close behind, so idle lanes waste issue slots and every
+    // refill stalls its wave on three dependent fetches: a large scene's long
+    // traversals want lanes refilled early, while a small scene's short ones
+    // finish so fast that each refill's fetches are better shared by more
+    // rays.  Measured from 2^17 triangles up, with the drain's
     // work sharing taking over the lanes idle at the end: a query is best at
     // 28 (4M-triangle query 6.06-6.08 ms against 6.10-6.11 at 24, 6.12-6.14
     // at 32, 6.27 at 20), a render's bounces at 24 (1080p render 32.7 ms
